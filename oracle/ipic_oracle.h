@@ -1,0 +1,133 @@
+/*
+ * ipic_oracle.h -- FP64 CPU restatement of the AllScale iPIC3D particle-in-cell step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This oracle is the checker for the GPU library: only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.  The product path
+ * (allscale_ipic3d_amd, libipic_gpu.so) never links, loads or calls it.
+ *
+ * Every function cites the reference file:line it restates (paths relative to
+ * code/app/include/ipic3d/app/ unless stated otherwise).  The composed step follows the
+ * intended phase order of simulator.h:189-223 (deposit -> field solve -> move -> rebin) with
+ * these explicit composition decisions:
+ *   D1 periodic boundaries (position wrap as src/pic_tracer.cxx:130-133, neighbour wrap
+ *      as cell.h:1012-1048);
+ *   D2 two-pass leapfrog (all E, then all Bc) followed by interpC2N so the mover sees the
+ *      updated node B;
+ *   D3 the physics grid is N-periodic: E, node B and J live on N^3 nodes, Bc on N^3 centres;
+ *   D4 J = sum(q v w) / vol / 8 (cell.h:822-823), no second aggregation stage;
+ *   D6 a particle stays in its cell while its offset lies in the closed interval [0,1]
+ *      (the reference's |relPos| <= halfWidth, cell.h:1134-1143); multi-cell moves are
+ *      resolved by repeating the one-cell rule;
+ *   D7 q and q/m are per particle, as in Particle (particle.h:9-16).
+ */
+#ifndef IPIC_ORACLE_H
+#define IPIC_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* particle.h:9-16 -- 8 doubles, 64 B, absolute position */
+typedef struct {
+    double pos[3];
+    double vel[3];
+    double q;
+    double qom;
+} or_particle;
+
+typedef struct {
+    int64_t n[3];            /* cells per axis */
+    double w[3];             /* cell width */
+    double origin[3];        /* physical origin of cell (0,0,0) */
+    double dt;
+    double c;                /* speed of light */
+    int32_t field_substeps;  /* leapfrog substeps per particle step (J held fixed), >= 1 */
+    int32_t sink;            /* 1: remove particles with |x - sink_center|^2 <= r^2 */
+    double sink_center[3];
+    double sink_radius;
+    int32_t bc;              /* 0: periodic (D1); 1: the reference's reflection at the global
+                                faces, cell.h:1103-1121 (shift by half a cell, negate v) */
+    int32_t pad_;
+} or_props;
+
+typedef struct or_universe or_universe;
+
+/* ---- single-particle / stencil kernels (reference KATs) ---- */
+void or_update_position(or_particle* p, double dt);                                 /* particle.h:24-26 */
+void or_update_velocity(or_particle* p, const double E[3], const double B[3], double dt); /* particle.h:31-49 */
+/* corners[i][j][k] flattened as corners[(i*2+j)*2+k]; pos fractional in [0,1]^3 */
+double or_trilinear(const double corners[8], const double pos[3], double vol);      /* cell.h:867-885 */
+
+/* Reference-layout helpers.  Arrays are dense [dx][dy][dz][3] grids of doubles. */
+void or_ref_curlB(const int64_t dims[3], const double* Bc, const int64_t pos[3],
+                  const double w[3], double curl[3]);                                /* field.h:162-193 */
+void or_ref_curlE(const int64_t dims[3], const double* E, const int64_t pos[3],
+                  const double w[3], double curl[3]);                                /* field.h:198-229 */
+void or_ref_interpC2N(const int64_t bc_dims[3], const double* Bc, const int64_t node_dims[3],
+                      double* B, const int64_t pos[3]);                              /* field.h:234-247 */
+void or_ref_interpN2C(const int64_t node_dims[3], const double* B, const int64_t bc_dims[3],
+                      double* Bc, const int64_t pos[3]);                             /* field.h:252-265 */
+/* field.h:407-440: face-only ghost copy with period end-1 (end = size-1) */
+void or_ref_update_boundaries(int64_t field_size, double* E, double* B,
+                              int64_t bc_size, double* Bc);
+/* field.h:342-372, one node, in place, reference indexing (E/B: field dims, Bc: bc dims, J: j dims) */
+void or_ref_leapfrog_node(const int64_t fdims[3], double* E, const int64_t bdims[3], double* Bc,
+                          const int64_t jdims[3], const double* J, const int64_t pos[3],
+                          const double w[3], double dt, double c);
+/* field.h:43-78: returns E, Bext, B at a location */
+void or_dipole_field(const double location[3], const double drift[3], const double B0[3],
+                     const double center[3], double planet_radius, double B1z,
+                     double E[3], double Bext[3], double B[3]);
+
+/* ---- universe ---- */
+or_universe* or_create(const or_props* props);
+void or_destroy(or_universe* u);
+int64_t or_num_cells(const or_universe* u);
+/* Bin particles into cells by floor((x - origin)/w); positions outside the box are wrapped. */
+int or_add_particles(or_universe* u, int64_t n, const or_particle* ps);
+int64_t or_num_particles(const or_universe* u);
+/* Copies all particles in cell order (cell linear index x-major: (x*ny + y)*nz + z),
+ * keeping each cell's internal order; cell_of[i] receives the cell index (may be NULL). */
+void or_get_particles(const or_universe* u, or_particle* out, int64_t* cell_of);
+void or_cell_counts(const or_universe* u, int64_t* counts);
+/* Fields: E, Bc, Bn, J are [ncell][3] with node/centre index (x*ny+y)*nz+z. */
+void or_set_fields(or_universe* u, const double* E, const double* Bc);
+void or_get_fields(const or_universe* u, double* E, double* Bc, double* Bn, double* J);
+void or_set_J(or_universe* u, const double* J);
+
+/* ---- step phases (N-periodic) ---- */
+void or_deposit(or_universe* u, int nthreads);      /* cell.h:787-824 */
+void or_field_solve(or_universe* u, int nthreads);  /* field.h:356-372 two-pass + field.h:234-247 */
+void or_move(or_universe* u, int nthreads);         /* cell.h:896-964 with gather cell.h:905-937 */
+int64_t or_rebin(or_universe* u, int nthreads);     /* cell.h:975-1159 + 1195-1310; returns #removed */
+/* simulator.h:181-223 intended loop: deposit, field, move, rebin per step */
+void or_step(or_universe* u, int64_t nsteps, int nthreads);
+
+/* ---- FP64 reductions ---- */
+double or_kinetic_energy(const or_universe* u);     /* cell.h:1315-1324 + 1346-1355 */
+double or_momentum(const or_universe* u);           /* cell.h:1329-1338 */
+double or_total_charge(const or_universe* u);
+/* W_E = sum 0.5|E|^2 vol, W_B = sum 0.5 c^2 |Bc|^2 vol over the N^3 nodes / centres */
+void or_field_energy(const or_universe* u, double* wE, double* wB);
+/* discrete forward divergence of Bc on the index lattice: returns max |div|, sets l2 */
+double or_divB(const or_universe* u, double* l2);
+/* field.h:443-459 form: sum vol/2 * |F|^2 / (4 pi) over nodes, F = E (which=0) or node B (1) */
+double or_ref_field_energy(const or_universe* u, int which);
+
+/* ---- synthetic initialisation shared with the GPU init kernel ---- */
+uint64_t or_mix64(uint64_t z);
+/* Draw particle `index` of (species, cell) for the given seed: offsets u[3] in [0,1),
+ * standard normals nrm[3] (Box-Muller on 53-bit uniforms). */
+void or_draw(uint64_t seed, int32_t species, int64_t cell, int64_t index,
+             double u[3], double nrm[3]);
+/* Fill `count` particles per cell (counts[cell]) with uniform offsets and Maxwellian
+ * velocities drift + vth*N(0,1), charge q and qom; appends to the universe. */
+int or_init_species(or_universe* u, int32_t species, uint64_t seed, const int64_t* counts,
+                    double q, double qom, const double vth[3], const double drift[3]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
