@@ -1,0 +1,264 @@
+"""MI355X-native particle-in-cell step of the AllScale iPIC3D pilot.
+
+The compute path is the HIP library ``_lib/libipic_gpu.so`` (C ABI in include/ipic_gpu.h).
+This module binds it with ctypes.  There is no CPU fallback: if the library is missing or
+no GPU is present, every call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_lib", "libipic_gpu.so")
+MAX_SPECIES = 8
+
+FLAG_TRACK_IDS = 1
+FLAG_NO_GRAPH = 2
+FLAG_PHASE_TIMING = 4
+
+
+class SpeciesDesc(C.Structure):
+    _fields_ = [("q", C.c_double), ("qom", C.c_double)]
+
+
+class Desc(C.Structure):
+    _fields_ = [
+        ("n", C.c_int32 * 3),
+        ("nspecies", C.c_int32),
+        ("cell_width", C.c_double * 3),
+        ("origin", C.c_double * 3),
+        ("dt", C.c_double),
+        ("c", C.c_double),
+        ("species", SpeciesDesc * MAX_SPECIES),
+        ("field_substeps", C.c_int32),
+        ("sink", C.c_int32),
+        ("sink_center", C.c_double * 3),
+        ("sink_radius", C.c_double),
+        ("tile", C.c_int32 * 3),
+        ("flags", C.c_uint32),
+    ]
+
+
+class StepStats(C.Structure):
+    _fields_ = [
+        ("first_step_s", C.c_double),
+        ("remaining_steps_s", C.c_double),
+        ("steps", C.c_int64),
+        ("particles", C.c_int64),
+        ("removed", C.c_int64),
+        ("ms_field", C.c_double),
+        ("ms_push", C.c_double),
+        ("ms_scan", C.c_double),
+        ("ms_deposit", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Conservation(C.Structure):
+    _fields_ = [
+        ("kinetic_energy", C.c_double),
+        ("momentum", C.c_double),
+        ("charge", C.c_double),
+        ("field_energy_E", C.c_double),
+        ("field_energy_B", C.c_double),
+        ("divB_max", C.c_double),
+        ("divB_l2", C.c_double),
+        ("EdotJ", C.c_double),
+        ("particles", C.c_int64),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+# every symbol declared in include/ipic_gpu.h
+ABI_SYMBOLS = (
+    "ipic_gpu_create", "ipic_gpu_destroy", "ipic_gpu_last_error", "ipic_gpu_upload_particles",
+    "ipic_gpu_init_particles", "ipic_gpu_upload_fields", "ipic_gpu_step", "ipic_gpu_num_particles",
+    "ipic_gpu_cell_counts", "ipic_gpu_download_particles", "ipic_gpu_download_fields",
+    "ipic_gpu_conservation", "ipic_gpu_device_info",
+)
+
+_lib = None
+
+
+class IpicError(RuntimeError):
+    pass
+
+
+def load_library(path: str = LIB_PATH):
+    """Loads libipic_gpu.so and declares the C ABI; raises if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise IpicError(f"HIP library not built: {path} (run python -m allscale_ipic3d_amd.build)")
+    L = C.CDLL(path)
+    P, I32, I64, U64 = C.c_void_p, C.c_int32, C.c_int64, C.c_uint64
+    L.ipic_gpu_create.argtypes = [C.POINTER(Desc), C.POINTER(C.c_void_p)]
+    L.ipic_gpu_destroy.argtypes = [P]
+    L.ipic_gpu_last_error.restype = C.c_char_p
+    L.ipic_gpu_last_error.argtypes = [P]
+    L.ipic_gpu_upload_particles.argtypes = [P, I32, I64, P, P]
+    L.ipic_gpu_init_particles.argtypes = [P, I32, P, I32, P, P, U64]
+    L.ipic_gpu_upload_fields.argtypes = [P, P, P]
+    L.ipic_gpu_step.argtypes = [P, I64, C.POINTER(StepStats)]
+    L.ipic_gpu_num_particles.argtypes = [P, I32, C.POINTER(C.c_int64)]
+    L.ipic_gpu_cell_counts.argtypes = [P, I32, P]
+    L.ipic_gpu_download_particles.argtypes = [P, I32, P, P, P, P]
+    L.ipic_gpu_download_fields.argtypes = [P, P, P, P, P]
+    L.ipic_gpu_conservation.argtypes = [P, C.POINTER(Conservation)]
+    L.ipic_gpu_device_info.argtypes = [C.c_char_p, I32, C.POINTER(C.c_int32), C.POINTER(C.c_int64)]
+    for name in ABI_SYMBOLS:
+        if name != "ipic_gpu_last_error":
+            getattr(L, name).restype = C.c_int
+    _lib = L
+    return L
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _f64(a, shape=None):
+    a = np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+    if shape is not None:
+        a = a.reshape(shape)
+    return a
+
+
+def device_info():
+    L = load_library()
+    name = C.create_string_buffer(256)
+    cu = C.c_int32()
+    mem = C.c_int64()
+    rc = L.ipic_gpu_device_info(name, 256, C.byref(cu), C.byref(mem))
+    if rc != 0:
+        raise IpicError(L.ipic_gpu_last_error(None).decode())
+    return {"name": name.value.decode(), "cu_count": cu.value, "hbm_bytes": mem.value}
+
+
+class Simulation:
+    """A universe resident on the GPU (the Universe of universe.h:12-51 behind the C ABI)."""
+
+    def __init__(self, n, cell_width=(1.0, 1.0, 1.0), origin=(0.0, 0.0, 0.0), dt=0.1, c=1.0,
+                 species=((-1.0, -1.0),), field_substeps=1, sink_center=None, sink_radius=0.0,
+                 tile=(0, 0, 0), track_ids=False, flags=0):
+        L = load_library()
+        if np.isscalar(n):
+            n = (n, n, n)
+        d = Desc()
+        d.n[:] = [int(v) for v in n]
+        d.nspecies = len(species)
+        d.cell_width[:] = [float(v) for v in cell_width]
+        d.origin[:] = [float(v) for v in origin]
+        d.dt, d.c = float(dt), float(c)
+        for i, (q, qom) in enumerate(species):
+            d.species[i].q = float(q)
+            d.species[i].qom = float(qom)
+        d.field_substeps = int(field_substeps)
+        if sink_center is not None:
+            d.sink = 1
+            d.sink_center[:] = [float(v) for v in sink_center]
+            d.sink_radius = float(sink_radius)
+        d.tile[:] = [int(v) for v in tile]
+        d.flags = int(flags) | (FLAG_TRACK_IDS if track_ids else 0)
+        self.desc = d
+        self.n = tuple(int(v) for v in n)
+        self.ncell = self.n[0] * self.n[1] * self.n[2]
+        self.nspecies = len(species)
+        self.track_ids = bool(track_ids)
+        h = C.c_void_p()
+        rc = L.ipic_gpu_create(C.byref(d), C.byref(h))
+        if rc != 0:
+            raise IpicError(f"ipic_gpu_create failed ({rc}): {L.ipic_gpu_last_error(None).decode()}")
+        self._h = h
+        self._L = L
+
+    def _check(self, rc):
+        if rc != 0:
+            raise IpicError(f"libipic_gpu error {rc}: {self._L.ipic_gpu_last_error(self._h).decode()}")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.ipic_gpu_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ---- state in
+    def upload_particles(self, species, pos, vel):
+        pos = _f64(pos, (-1, 3))
+        vel = _f64(vel, (-1, 3))
+        assert len(pos) == len(vel)
+        self._check(self._L.ipic_gpu_upload_particles(self._h, int(species), len(pos), _ptr(pos), _ptr(vel)))
+
+    def init_particles(self, species, ppc, vth, drift=(0.0, 0.0, 0.0), seed=0):
+        if np.isscalar(ppc):
+            arr, uni = None, int(ppc)
+        else:
+            arr = np.ascontiguousarray(np.asarray(ppc, dtype=np.int32).reshape(-1))
+            assert arr.size == self.ncell
+            uni = 0
+        self._check(self._L.ipic_gpu_init_particles(self._h, int(species), _ptr(arr), uni, _ptr(_f64(vth)),
+                                                    _ptr(_f64(drift)), int(seed)))
+
+    def upload_fields(self, E=None, Bc=None):
+        e = None if E is None else _f64(E, (self.ncell, 3))
+        b = None if Bc is None else _f64(Bc, (self.ncell, 3))
+        self._check(self._L.ipic_gpu_upload_fields(self._h, _ptr(e), _ptr(b)))
+
+    # ---- stepping
+    def step(self, nsteps=1):
+        st = StepStats()
+        self._check(self._L.ipic_gpu_step(self._h, int(nsteps), C.byref(st)))
+        return st.as_dict()
+
+    # ---- state out
+    def num_particles(self, species=-1):
+        n = C.c_int64()
+        self._check(self._L.ipic_gpu_num_particles(self._h, int(species), C.byref(n)))
+        return n.value
+
+    def cell_counts(self, species=-1):
+        out = np.zeros(self.ncell, dtype=np.int64)
+        self._check(self._L.ipic_gpu_cell_counts(self._h, int(species), _ptr(out)))
+        return out
+
+    def download_particles(self, species):
+        n = self.num_particles(species)
+        pos = np.zeros((n, 3))
+        vel = np.zeros((n, 3))
+        cell = np.zeros(n, dtype=np.int64)
+        ids = np.zeros(n, dtype=np.uint32) if self.track_ids else None
+        self._check(self._L.ipic_gpu_download_particles(self._h, int(species), _ptr(pos), _ptr(vel), _ptr(cell),
+                                                        _ptr(ids)))
+        out = {"pos": pos, "vel": vel, "cell": cell}
+        if ids is not None:
+            out["ids"] = ids
+        return out
+
+    def download_fields(self):
+        E, Bc, Bn, J = (np.zeros((self.ncell, 3)) for _ in range(4))
+        self._check(self._L.ipic_gpu_download_fields(self._h, _ptr(E), _ptr(Bc), _ptr(Bn), _ptr(J)))
+        return {"E": E, "Bc": Bc, "Bn": Bn, "J": J}
+
+    def conservation(self):
+        c = Conservation()
+        self._check(self._L.ipic_gpu_conservation(self._h, C.byref(c)))
+        return c.as_dict()

@@ -1,0 +1,1844 @@
+// ipic_gpu.hip -- MI355X (gfx950) particle-in-cell step behind the C ABI of include/ipic_gpu.h.
+//
+// Device layout (one HIP stream per context):
+//   particles  per species, structure of arrays of FP32: cell-relative offsets ox,oy,oz (cell
+//              units) and velocities vx,vy,vz; 24 B per particle; sorted by cell in tile-major
+//              cell order (cid = tile * TC + local cell), ping-pong buffers X[0], X[1].
+//   fields     E (nodes), Bc (centres), Bn (nodes), J (nodes) as float4 rows, row-major.
+//   J partials per tile: the tile's nodes plus a one-node halo, (T+3)^3 x 3 FP32, summed
+//              across tiles in FP64 by the E pass.
+//
+// The particle array lags its sort key by one step: X_n is sorted by the cell each particle
+// occupied when the previous push began (its "segment" a) and stores offsets relative to a,
+// so its current cell is b = a + dcell(offset).  One pass over X_n (k_particles<PUSH>)
+// gathers E,B at b, pushes, writes the particle into X_{n+1} sorted by b (positions known
+// from the counts gathered by the previous pass), deposits J at the new cell e, and counts
+// particles per e for the next pass.  Every particle is read once and written once per step.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "ipic_gpu.h"
+
+namespace {
+
+constexpr int NT = 256;          // threads per block of the particle kernels
+constexpr int KRUN = 8;          // max particles per run (one lane)
+constexpr int CH = NT * KRUN;    // max particles staged in LDS per chunk
+constexpr int TCMAX = 512;       // max cells per tile
+constexpr int NJMAX = 11 * 11 * 11;  // max J nodes per tile ((T+3)^3, T <= 8)
+constexpr int JPMAX = 4;         // max (tile, local) pairs per axis coordinate
+
+enum { MODE_PUSH = 0, MODE_DEPOSIT = 1, MODE_REBIN = 2 };
+
+struct Geo {
+    int n[3];
+    int T[3];
+    int NTl[3];
+    int TC;
+    int ntiles;
+    int ncell;
+    int NJ;
+    int nj[3];
+    int nmax;
+    float w[3];
+    float inv_w[3];
+    float origin[3];
+    float dt;
+    float inv_vol;
+    float pi_c;
+    int sink;
+    float sink_c[3];
+    float sink_r2;
+    float E0[3];          // uniform part of the background field (added in the gather)
+    float B0[3];
+};
+
+struct SpArgs {
+    const float* in;      // 6 arrays of cap floats: ox oy oz vx vy vz
+    float* out;
+    const uint32_t* id_in;
+    uint32_t* id_out;
+    long long cap;
+    const int* seg;       // segment starts of `in`, ncell+1
+    const int* start;     // output starts by current cell, ncell+1
+    const int* stay;      // particles of segment c that are in cell c
+    int* arr;             // arrival counters per cell
+    int* cnt_acc;         // next-step counts by new cell
+    int* stay_acc;        // next-step stayers by output segment
+    float qom;
+    float qs;             // q / (8 vol)
+};
+
+struct PushArgs {
+    Geo g;
+    SpArgs sp[IPIC_MAX_SPECIES];
+    int nsp;
+    const float4* E;
+    const float4* Bn;
+    float* jpart;
+    float* jover;
+    int* jflag;
+    unsigned long long* removed;
+    int* err;             // set to nonzero when a consistency guard fires
+};
+
+// ---------------------------------------------------------------------------------------------
+// device helpers
+// ---------------------------------------------------------------------------------------------
+
+// One-cell rule of cell.h:1134-1143 in offset form (a particle stays while its offset lies in
+// the closed interval [0,1]), repeated for moves of more than one cell.
+__device__ __forceinline__ int dcell(float o) {
+    if (o > 1.0f) return (int)fminf(ceilf(o - 1.0f), 1048576.0f);
+    if (o < 0.0f) return (int)fmaxf(floorf(o), -1048576.0f);
+    return 0;
+}
+
+__device__ __forceinline__ int wrapc(int i, int n) {
+    int r = i % n;
+    return r < 0 ? r + n : r;
+}
+
+__device__ __forceinline__ int cid_of(const Geo& g, int x, int y, int z) {
+    const int tx = x / g.T[0], ty = y / g.T[1], tz = z / g.T[2];
+    const int lx = x - tx * g.T[0], ly = y - ty * g.T[1], lz = z - tz * g.T[2];
+    return ((tx * g.NTl[1] + ty) * g.NTl[2] + tz) * g.TC + (lx * g.T[1] + ly) * g.T[2] + lz;
+}
+
+__device__ __forceinline__ int nid_of(const Geo& g, int x, int y, int z) {
+    return (x * g.n[1] + y) * g.n[2] + z;
+}
+
+// XCD-aware block -> tile map: consecutive tiles (neighbours in x-slabs) go to the same XCD
+// (blocks are dealt round-robin over the 8 XCDs), so halo nodes shared by neighbouring tiles
+// stay in one L2.
+__device__ __forceinline__ int xcd_tile(int bid, int ntiles) {
+    const int per = ntiles / 8, rem = ntiles % 8;
+    const int xcd = bid % 8, slot = bid / 8;
+    return xcd < rem ? xcd * (per + 1) + slot : rem * (per + 1) + (xcd - rem) * per + slot;
+}
+
+// Exclusive prefix sum over the block (NT threads); returns the prefix, sets total.
+__device__ __forceinline__ int block_exscan(int v, int* s_w, int& total) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    int x = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
+    }
+    if (lane == 63) s_w[wid] = x;
+    __syncthreads();
+    int off = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < NT / 64; ++w) {
+        const int sw = s_w[w];
+        off += (w < wid) ? sw : 0;
+        tot += sw;
+    }
+    total = tot;
+    __syncthreads();
+    return off + x - v;
+}
+
+__device__ __forceinline__ void lds_add(float* p, float v) {
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Trilinear weights (cell.h:876-880): w[(i*2+j)*2+k] = (i?x:1-x)(j?y:1-y)(k?z:1-z)
+__device__ __forceinline__ void tri_weights(float x, float y, float z, float wt[8]) {
+    const float x0 = 1.0f - x, y0 = 1.0f - y, z0 = 1.0f - z;
+    const float a00 = x0 * y0, a01 = x0 * y, a10 = x * y0, a11 = x * y;
+    wt[0] = a00 * z0; wt[1] = a00 * z;
+    wt[2] = a01 * z0; wt[3] = a01 * z;
+    wt[4] = a10 * z0; wt[5] = a10 * z;
+    wt[6] = a11 * z0; wt[7] = a11 * z;
+}
+
+// Deposit of one particle outside the run's own cell: corner (i,j,k) of cell e goes to the
+// tile's LDS accumulator when inside its halo, else to the global overflow accumulator.
+__device__ __forceinline__ void deposit_direct(const PushArgs& A, float* sJ, int x0, int y0, int z0,
+                                               int elx, int ely, int elz, const float wt[8], float jx,
+                                               float jy, float jz) {
+    const Geo& g = A.g;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                const float ww = wt[(i * 2 + j) * 2 + k];
+                const int nx = elx + 1 + i, ny = ely + 1 + j, nz = elz + 1 + k;
+                if (nx >= 0 && nx < g.nj[0] && ny >= 0 && ny < g.nj[1] && nz >= 0 && nz < g.nj[2]) {
+                    const int l = (nx * g.nj[1] + ny) * g.nj[2] + nz;
+                    lds_add(&sJ[l], jx * ww);
+                    lds_add(&sJ[g.NJ + l], jy * ww);
+                    lds_add(&sJ[2 * g.NJ + l], jz * ww);
+                } else {
+                    const int gid = nid_of(g, wrapc(x0 + elx + i, g.n[0]), wrapc(y0 + ely + j, g.n[1]),
+                                           wrapc(z0 + elz + k, g.n[2]));
+                    atomicAdd(&A.jover[3 * gid + 0], jx * ww);
+                    atomicAdd(&A.jover[3 * gid + 1], jy * ww);
+                    atomicAdd(&A.jover[3 * gid + 2], jz * ww);
+                    *A.jflag = 1;
+                }
+            }
+}
+
+__device__ __forceinline__ void count_add(const Geo& g, int* s_arr, int* gl, int x0, int y0, int z0,
+                                          int ex, int ey, int ez) {
+    const int lx = ex - x0, ly = ey - y0, lz = ez - z0;
+    if (lx >= 0 && lx < g.T[0] && ly >= 0 && ly < g.T[1] && lz >= 0 && lz < g.T[2]) {
+        atomicAdd(&s_arr[(lx * g.T[1] + ly) * g.T[2] + lz], 1);
+    } else {
+        atomicAdd(&gl[cid_of(g, wrapc(ex, g.n[0]), wrapc(ey, g.n[1]), wrapc(ez, g.n[2]))], 1);
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// particle kernel: PUSH (gather + Boris + rebin + deposit), DEPOSIT (J only), REBIN (sort only)
+// ---------------------------------------------------------------------------------------------
+
+template <int MODE, bool TRACK>
+__global__ __launch_bounds__(NT) void k_particles(PushArgs A) {
+    const Geo& g = A.g;
+    __shared__ float sP[6][CH];
+    __shared__ uint32_t sId[TRACK ? CH : 1];
+    __shared__ float sJ[3 * NJMAX];
+    __shared__ int s_seg[TCMAX + 1];
+    __shared__ int s_rb[TCMAX + 1];
+    __shared__ int s_scan[NT];
+    __shared__ int s_w[NT / 64];
+    __shared__ int s_carry[TCMAX];
+    __shared__ int s_cnt[TCMAX];
+    __shared__ int s_stay[TCMAX];
+    __shared__ int s_prange[2];
+
+    const int tid = threadIdx.x;
+    const int t = xcd_tile(blockIdx.x, g.ntiles);
+    const int tx = t / (g.NTl[1] * g.NTl[2]), ty = (t / g.NTl[2]) % g.NTl[1], tz = t % g.NTl[2];
+    const int x0 = tx * g.T[0], y0 = ty * g.T[1], z0 = tz * g.T[2];
+    const int cbase = t * g.TC;
+    const float NaNf = __builtin_nanf("");
+
+    if (MODE != MODE_REBIN)
+        for (int i = tid; i < 3 * g.NJ; i += NT) sJ[i] = 0.0f;
+
+    for (int s = 0; s < A.nsp; ++s) {
+        const SpArgs& S = A.sp[s];
+        for (int i = tid; i <= g.TC; i += NT) s_seg[i] = S.seg[cbase + i];
+        for (int i = tid; i < g.TC; i += NT) {
+            s_carry[i] = 0;
+            s_cnt[i] = 0;
+            s_stay[i] = 0;
+        }
+        __syncthreads();
+        // runs: each cell's segment is split into ceil(n/KRUN) runs of balanced length
+        int r0 = 0, r1 = 0;
+        const int c0 = 2 * tid, c1 = 2 * tid + 1;
+        if (c0 < g.TC) r0 = (s_seg[c0 + 1] - s_seg[c0] + KRUN - 1) / KRUN;
+        if (c1 < g.TC) r1 = (s_seg[c1 + 1] - s_seg[c1] + KRUN - 1) / KRUN;
+        int R;
+        const int ex0 = block_exscan(r0 + r1, s_w, R);
+        if (c0 < g.TC) s_rb[c0] = ex0;
+        if (c1 < g.TC) s_rb[c1] = ex0 + r0;
+        if (tid == 0) s_rb[g.TC] = R;
+        __syncthreads();
+
+        for (int gq = 0; gq < R; gq += NT) {
+            const int q = gq + tid;
+            const bool active = q < R;
+            int c = 0, len = 0, pst = 0;
+            if (active) {
+                int lo = 0, hi = g.TC - 1;
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (s_rb[mid] <= q) lo = mid; else hi = mid - 1;
+                }
+                c = lo;
+                const int j = q - s_rb[c];
+                const int nr = s_rb[c + 1] - s_rb[c];
+                const int nc = s_seg[c + 1] - s_seg[c];
+                const int bl = nc / nr, rem = nc - bl * nr;
+                len = bl + (j < rem ? 1 : 0);
+                pst = s_seg[c] + j * bl + min(j, rem);
+            }
+            if (tid == 0) s_prange[0] = pst;
+            if (active && (q + 1 == R || tid == NT - 1)) s_prange[1] = pst + len;
+            __syncthreads();
+            const int plo = s_prange[0], nch = s_prange[1] - plo;
+            if (plo < 0 || (long long)plo + nch > S.cap || nch > CH) {
+                if (tid == 0) atomicOr(A.err, 2);
+                break;  // uniform over the block
+            }
+            for (int i = tid; i < nch; i += NT) {
+#pragma unroll
+                for (int k = 0; k < 6; ++k) sP[k][i] = S.in[(long long)k * S.cap + plo + i];
+                if (TRACK) sId[i] = S.id_in[plo + i];
+            }
+            __syncthreads();
+
+            // pass A: stayers of this run (particles of segment a still in cell a)
+            const int lb = pst - plo;
+            int my_stay = 0;
+            if (active && MODE != MODE_DEPOSIT) {
+                for (int jj = 0; jj < len; ++jj) {
+                    const float ox = sP[0][lb + jj], oy = sP[1][lb + jj], oz = sP[2][lb + jj];
+                    if (ox == ox && dcell(ox) == 0 && dcell(oy) == 0 && dcell(oz) == 0) ++my_stay;
+                }
+            }
+            int tot_stay;
+            const int sex = block_exscan(my_stay, s_w, tot_stay);
+            s_scan[tid] = sex;
+            __syncthreads();
+            int base = 0;
+            if (active && MODE != MODE_DEPOSIT) {
+                const int ft = max(s_rb[c], gq) - gq;
+                base = s_carry[c] + sex - s_scan[ft];
+            }
+            __syncthreads();
+            if (active && MODE != MODE_DEPOSIT) {
+                if (q + 1 == s_rb[c + 1] || tid == NT - 1 || q + 1 == R) s_carry[c] = base + my_stay;
+            }
+
+            // pass B: process the run
+            if (active) {
+                const int lx = c / (g.T[1] * g.T[2]), ly = (c / g.T[2]) % g.T[1], lz = c % g.T[2];
+                const int ax = x0 + lx, ay = y0 + ly, az = z0 + lz;
+                float4 cE[8], cB[8];
+                if (MODE == MODE_PUSH) {
+#pragma unroll
+                    for (int i = 0; i < 2; ++i)
+#pragma unroll
+                        for (int j = 0; j < 2; ++j)
+#pragma unroll
+                            for (int k = 0; k < 2; ++k) {
+                                const int nd = nid_of(g, wrapc(ax + i, g.n[0]), wrapc(ay + j, g.n[1]),
+                                                      wrapc(az + k, g.n[2]));
+                                cE[(i * 2 + j) * 2 + k] = A.E[nd];
+                                cB[(i * 2 + j) * 2 + k] = A.Bn[nd];
+                            }
+                }
+                float acc[8][3];
+#pragma unroll
+                for (int m = 0; m < 8; ++m) acc[m][0] = acc[m][1] = acc[m][2] = 0.0f;
+                bool acc_used = false;
+                int run_cnt = 0, run_stay = 0, rank = 0;
+                const int a_start = (MODE != MODE_DEPOSIT) ? S.start[cbase + c] : 0;
+                const int a_end = (MODE != MODE_DEPOSIT) ? S.start[cbase + c + 1] : 0;
+                const float qom = S.qom, qs = S.qs;
+
+                for (int jj = 0; jj < len; ++jj) {
+                    const int i = lb + jj;
+                    const float ox = sP[0][i], oy = sP[1][i], oz = sP[2][i];
+                    if (!(ox == ox)) continue;  // slot of a particle removed by the sink
+                    float vx = sP[3][i], vy = sP[4][i], vz = sP[5][i];
+                    const int dbx = dcell(ox), dby = dcell(oy), dbz = dcell(oz);
+                    const bool moved = (dbx | dby | dbz) != 0;
+                    float px = ox - (float)dbx, py = oy - (float)dby, pz = oz - (float)dbz;
+                    const int bx = ax + dbx, by = ay + dby, bz = az + dbz;  // unwrapped
+
+                    if (MODE == MODE_PUSH) {
+                        float wt[8];
+                        tri_weights(px, py, pz, wt);
+                        float Ex = 0, Ey = 0, Ez = 0, Bx = 0, By = 0, Bz = 0;
+                        if (!moved) {
+#pragma unroll
+                            for (int m = 0; m < 8; ++m) {
+                                Ex += cE[m].x * wt[m]; Ey += cE[m].y * wt[m]; Ez += cE[m].z * wt[m];
+                                Bx += cB[m].x * wt[m]; By += cB[m].y * wt[m]; Bz += cB[m].z * wt[m];
+                            }
+                        } else {
+                            const int bxw = wrapc(bx, g.n[0]), byw = wrapc(by, g.n[1]), bzw = wrapc(bz, g.n[2]);
+#pragma unroll
+                            for (int ii = 0; ii < 2; ++ii)
+#pragma unroll
+                                for (int jj2 = 0; jj2 < 2; ++jj2)
+#pragma unroll
+                                    for (int kk = 0; kk < 2; ++kk) {
+                                        const int nd = nid_of(g, wrapc(bxw + ii, g.n[0]), wrapc(byw + jj2, g.n[1]),
+                                                              wrapc(bzw + kk, g.n[2]));
+                                        const float4 e4 = A.E[nd], b4 = A.Bn[nd];
+                                        const float ww = wt[(ii * 2 + jj2) * 2 + kk];
+                                        Ex += e4.x * ww; Ey += e4.y * ww; Ez += e4.z * ww;
+                                        Bx += b4.x * ww; By += b4.y * ww; Bz += b4.z * ww;
+                                    }
+                        }
+                        // background uniform part (weights sum to one), then the division by the
+                        // cell volume of trilinearInterpolationF2P (cell.h:884)
+                        Ex += g.E0[0]; Ey += g.E0[1]; Ez += g.E0[2];
+                        Bx += g.B0[0]; By += g.B0[1]; Bz += g.B0[2];
+                        Ex *= g.inv_vol; Ey *= g.inv_vol; Ez *= g.inv_vol;
+                        Bx *= g.inv_vol; By *= g.inv_vol; Bz *= g.inv_vol;
+                        // adaptive sub-cycling (cell.h:948-952)
+                        const float Bmag = Bx * Bx + By * By + Bz * Bz;
+                        const float dt_sub0 = g.pi_c / (4.0f * fabsf(qom) * Bmag);
+                        const float ratio = g.dt / dt_sub0;
+                        const int nsub = (ratio < 99.0f) ? (int)ratio + 1 : 100;
+                        const float dts = g.dt / (float)nsub;
+                        const float kq = qom * 0.5f * dts;
+                        const float sx = dts * g.inv_w[0], sy = dts * g.inv_w[1], sz = dts * g.inv_w[2];
+                        for (int sc = 0; sc < nsub; ++sc) {
+                            // Boris rotation (particle.h:31-49)
+                            const float tx_ = kq * Bx, ty_ = kq * By, tz_ = kq * Bz;
+                            const float t2 = tx_ * tx_ + ty_ * ty_ + tz_ * tz_;
+                            const float f = 2.0f / (1.0f + t2);
+                            const float sx_ = f * tx_, sy_ = f * ty_, sz_ = f * tz_;
+                            const float mx = vx + kq * Ex, my = vy + kq * Ey, mz = vz + kq * Ez;
+                            const float qx = mx + (my * tz_ - mz * ty_);
+                            const float qy = my + (mz * tx_ - mx * tz_);
+                            const float qz = mz + (mx * ty_ - my * tx_);
+                            vx = mx + (qy * sz_ - qz * sy_) + kq * Ex;
+                            vy = my + (qz * sx_ - qx * sz_) + kq * Ey;
+                            vz = mz + (qx * sy_ - qy * sx_) + kq * Ez;
+                            // updatePosition (particle.h:24-26) in cell units
+                            px += vx * sx;
+                            py += vy * sy;
+                            pz += vz * sz;
+                        }
+                    }
+
+                    // cell after the move (relative to b) and offsets within it
+                    int ex = bx, ey = by, ez = bz;
+                    float qx_ = px, qy_ = py, qz_ = pz;
+                    bool sunk = false;
+                    if (MODE == MODE_PUSH) {
+                        const int dex = dcell(px), dey = dcell(py), dez = dcell(pz);
+                        ex += dex; ey += dey; ez += dez;
+                        qx_ -= (float)dex; qy_ -= (float)dey; qz_ -= (float)dez;
+                        if (g.sink) {
+                            const float ax_ = g.origin[0] + ((float)wrapc(ex, g.n[0]) + qx_) * g.w[0] - g.sink_c[0];
+                            const float ay_ = g.origin[1] + ((float)wrapc(ey, g.n[1]) + qy_) * g.w[1] - g.sink_c[1];
+                            const float az_ = g.origin[2] + ((float)wrapc(ez, g.n[2]) + qz_) * g.w[2] - g.sink_c[2];
+                            sunk = (ax_ * ax_ + ay_ * ay_ + az_ * az_) <= g.sink_r2;
+                        }
+                    }
+
+                    if (MODE != MODE_DEPOSIT) {
+                        int oidx, olim;
+                        if (!moved) {
+                            oidx = a_start + base + rank;
+                            olim = a_end;
+                            ++rank;
+                        } else {
+                            const int bc = cid_of(g, wrapc(bx, g.n[0]), wrapc(by, g.n[1]), wrapc(bz, g.n[2]));
+                            oidx = S.start[bc] + S.stay[bc] + atomicAdd(&S.arr[bc], 1);
+                            olim = S.start[bc + 1];
+                        }
+                        // guard: a slot outside the cell's range means the per-cell counts are
+                        // inconsistent; report instead of writing out of bounds
+                        if (oidx < 0 || oidx >= olim || (long long)oidx >= S.cap) {
+                            atomicOr(A.err, 1);
+                            continue;
+                        }
+                        float* o = S.out;
+                        o[oidx] = sunk ? NaNf : px;
+                        o[S.cap + oidx] = py;
+                        o[2 * S.cap + oidx] = pz;
+                        o[3 * S.cap + oidx] = vx;
+                        o[4 * S.cap + oidx] = vy;
+                        o[5 * S.cap + oidx] = vz;
+                        if (TRACK) S.id_out[oidx] = sId[i];
+                        if (MODE == MODE_PUSH) {
+                            if (sunk) {
+                                atomicAdd(A.removed, 1ull);
+                            } else {
+                                const bool e_is_b = (ex == bx) && (ey == by) && (ez == bz);
+                                if (!moved && e_is_b) {
+                                    ++run_cnt;
+                                    ++run_stay;
+                                } else {
+                                    count_add(g, s_cnt, S.cnt_acc, x0, y0, z0, ex, ey, ez);
+                                    if (e_is_b) count_add(g, s_stay, S.stay_acc, x0, y0, z0, bx, by, bz);
+                                }
+                            }
+                        }
+                    }
+
+                    if (MODE != MODE_REBIN && !sunk) {
+                        float wt[8];
+                        tri_weights(qx_, qy_, qz_, wt);
+                        const float jx = qs * vx, jy = qs * vy, jz = qs * vz;
+                        if (ex == ax && ey == ay && ez == az) {
+#pragma unroll
+                            for (int m = 0; m < 8; ++m) {
+                                acc[m][0] += jx * wt[m];
+                                acc[m][1] += jy * wt[m];
+                                acc[m][2] += jz * wt[m];
+                            }
+                            acc_used = true;
+                        } else {
+                            deposit_direct(A, sJ, x0, y0, z0, ex - x0, ey - y0, ez - z0, wt, jx, jy, jz);
+                        }
+                    }
+                }
+                if (MODE != MODE_REBIN && acc_used) {
+#pragma unroll
+                    for (int i = 0; i < 2; ++i)
+#pragma unroll
+                        for (int j = 0; j < 2; ++j)
+#pragma unroll
+                            for (int k = 0; k < 2; ++k) {
+                                const int l = ((lx + 1 + i) * g.nj[1] + (ly + 1 + j)) * g.nj[2] + (lz + 1 + k);
+                                const int m = (i * 2 + j) * 2 + k;
+                                lds_add(&sJ[l], acc[m][0]);
+                                lds_add(&sJ[g.NJ + l], acc[m][1]);
+                                lds_add(&sJ[2 * g.NJ + l], acc[m][2]);
+                            }
+                }
+                if (MODE == MODE_PUSH && (run_cnt | run_stay)) {
+                    atomicAdd(&s_cnt[c], run_cnt);
+                    atomicAdd(&s_stay[c], run_stay);
+                }
+            }
+            __syncthreads();
+        }
+        if (MODE == MODE_PUSH) {
+            for (int i = tid; i < g.TC; i += NT) {
+                if (s_cnt[i]) atomicAdd(&S.cnt_acc[cbase + i], s_cnt[i]);
+                if (s_stay[i]) atomicAdd(&S.stay_acc[cbase + i], s_stay[i]);
+            }
+        }
+        __syncthreads();
+    }
+    if (MODE != MODE_REBIN) {
+        float* dst = A.jpart + (long long)t * 3 * g.NJ;
+        for (int i = tid; i < 3 * g.NJ; i += NT) dst[i] = sJ[i];
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// field kernels (N-periodic, row-major float4)
+// ---------------------------------------------------------------------------------------------
+
+// The field is held as a static background (the state at upload) plus an FP32 perturbation
+// that the leapfrog evolves: E = E0 + Ebg + dE, Bc = B0 + Bcbg + dBc.  The uniform parts E0,
+// B0 are constants, the non-uniform parts are optional arrays, and the curls of the
+// background enter as static sources SE = c^2 curl- Bc_bg and SB = curl+ E_bg computed in
+// FP64 on the host.  A large uniform B0 therefore costs no precision in the curls.
+struct FieldArgs {
+    Geo g;
+    float4* E;            // dE on nodes
+    float4* Bc;           // dBc on centres
+    float4* Bn;           // gather B minus B0: interpC2N(dBc) + Bnbg
+    float4* J;
+    float4* Eg;           // gather E minus E0 (dE + Ebg) when Ebg exists, else null
+    const float4* Ebg;    // non-uniform background E on nodes (nullable)
+    const float4* Bcbg;   // non-uniform background Bc on centres (nullable)
+    const float4* Bnbg;   // interpC2N(Bcbg) on nodes (nullable)
+    const float4* SE;     // static E source (nullable)
+    const float4* SB;     // static B source (nullable)
+    const float* jpart;
+    float* jover;
+    int* jflag;
+    const int* jpairs;  // [3][nmax][1+JPMAX]: count, then (tile << 8 | local)
+    float dt;           // field substep
+    float c2;
+};
+
+__device__ __forceinline__ void xyz_of(const Geo& g, int i, int& x, int& y, int& z) {
+    x = i / (g.n[1] * g.n[2]);
+    y = (i / g.n[2]) % g.n[1];
+    z = i % g.n[2];
+}
+
+// E pass of field.h:359-363: E += dt (c^2 curl- Bc - J).  With from_partials the current
+// density of each node is first summed in FP64 over every tile partial covering it (the
+// cross-tile moment sum) plus the overflow accumulator, and stored to J.
+__global__ __launch_bounds__(256) void k_field_E(FieldArgs F, int from_partials) {
+    const Geo& g = F.g;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= g.ncell) return;
+    int x, y, z;
+    xyz_of(g, i, x, y, z);
+    float Jx, Jy, Jz;
+    if (from_partials) {
+        double sx = 0.0, sy = 0.0, sz = 0.0;
+        const int* px = F.jpairs + (0 * g.nmax + x) * (1 + JPMAX);
+        const int* py = F.jpairs + (1 * g.nmax + y) * (1 + JPMAX);
+        const int* pz = F.jpairs + (2 * g.nmax + z) * (1 + JPMAX);
+        for (int a = 0; a < px[0]; ++a)
+            for (int b = 0; b < py[0]; ++b)
+                for (int cc = 0; cc < pz[0]; ++cc) {
+                    const int ta = px[1 + a] >> 8, la = px[1 + a] & 255;
+                    const int tb = py[1 + b] >> 8, lb = py[1 + b] & 255;
+                    const int tc = pz[1 + cc] >> 8, lc = pz[1 + cc] & 255;
+                    const int tile = (ta * g.NTl[1] + tb) * g.NTl[2] + tc;
+                    const int l = (la * g.nj[1] + lb) * g.nj[2] + lc;
+                    const float* base = F.jpart + (long long)tile * 3 * g.NJ;
+                    sx += (double)base[l];
+                    sy += (double)base[g.NJ + l];
+                    sz += (double)base[2 * g.NJ + l];
+                }
+        if (*F.jflag) {
+            sx += (double)F.jover[3 * i + 0];
+            sy += (double)F.jover[3 * i + 1];
+            sz += (double)F.jover[3 * i + 2];
+            F.jover[3 * i + 0] = 0.0f;
+            F.jover[3 * i + 1] = 0.0f;
+            F.jover[3 * i + 2] = 0.0f;
+        }
+        Jx = (float)sx; Jy = (float)sy; Jz = (float)sz;
+        F.J[i] = make_float4(Jx, Jy, Jz, 0.0f);
+    } else {
+        const float4 j4 = F.J[i];
+        Jx = j4.x; Jy = j4.y; Jz = j4.z;
+    }
+    const float4 b = F.Bc[i];
+    const float4 bmx = F.Bc[nid_of(g, wrapc(x - 1, g.n[0]), y, z)];
+    const float4 bmy = F.Bc[nid_of(g, x, wrapc(y - 1, g.n[1]), z)];
+    const float4 bmz = F.Bc[nid_of(g, x, y, wrapc(z - 1, g.n[2]))];
+    float4 e = F.E[i];
+    const float dt = F.dt, c2 = F.c2;
+    e.x = e.x + dt * (c2 * ((b.z - bmy.z) * g.inv_w[1] - (b.y - bmz.y) * g.inv_w[2]) - Jx);
+    e.y = e.y + dt * (c2 * ((b.x - bmz.x) * g.inv_w[2] - (b.z - bmx.z) * g.inv_w[0]) - Jy);
+    e.z = e.z + dt * (c2 * ((b.y - bmx.y) * g.inv_w[0] - (b.x - bmy.x) * g.inv_w[1]) - Jz);
+    if (F.SE) {
+        const float4 se = F.SE[i];
+        e.x += dt * se.x; e.y += dt * se.y; e.z += dt * se.z;
+    }
+    F.E[i] = e;
+    if (F.Eg) {
+        const float4 eb = F.Ebg[i];
+        F.Eg[i] = make_float4(e.x + eb.x, e.y + eb.y, e.z + eb.z, 0.0f);
+    }
+}
+
+// B pass of field.h:366-371: Bc -= dt curl+ E.  Clears the overflow flag read by the E pass.
+__global__ __launch_bounds__(256) void k_field_B(FieldArgs F) {
+    const Geo& g = F.g;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) *F.jflag = 0;
+    if (i >= g.ncell) return;
+    int x, y, z;
+    xyz_of(g, i, x, y, z);
+    const float4 e = F.E[i];
+    const float4 epx = F.E[nid_of(g, wrapc(x + 1, g.n[0]), y, z)];
+    const float4 epy = F.E[nid_of(g, x, wrapc(y + 1, g.n[1]), z)];
+    const float4 epz = F.E[nid_of(g, x, y, wrapc(z + 1, g.n[2]))];
+    float4 b = F.Bc[i];
+    const float dt = F.dt;
+    b.x = b.x - dt * ((epy.z - e.z) * g.inv_w[1] - (epz.y - e.y) * g.inv_w[2]);
+    b.y = b.y - dt * ((epz.x - e.x) * g.inv_w[2] - (epx.z - e.z) * g.inv_w[0]);
+    b.z = b.z - dt * ((epx.y - e.y) * g.inv_w[0] - (epy.x - e.x) * g.inv_w[1]);
+    if (F.SB) {
+        const float4 sb = F.SB[i];
+        b.x -= dt * sb.x; b.y -= dt * sb.y; b.z -= dt * sb.z;
+    }
+    F.Bc[i] = b;
+}
+
+// interpC2N (field.h:234-247): node B = 1/8 sum of the 8 surrounding centres.
+__global__ __launch_bounds__(256) void k_c2n(FieldArgs F) {
+    const Geo& g = F.g;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= g.ncell) return;
+    int x, y, z;
+    xyz_of(g, i, x, y, z);
+    float sx = 0, sy = 0, sz = 0;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                const float4 v = F.Bc[nid_of(g, wrapc(x - a, g.n[0]), wrapc(y - b, g.n[1]), wrapc(z - c, g.n[2]))];
+                sx += v.x; sy += v.y; sz += v.z;
+            }
+    float4 r = make_float4(0.125f * sx, 0.125f * sy, 0.125f * sz, 0.0f);
+    if (F.Bnbg) {
+        const float4 bb = F.Bnbg[i];
+        r.x += bb.x; r.y += bb.y; r.z += bb.z;
+    }
+    F.Bn[i] = r;
+}
+
+// ---------------------------------------------------------------------------------------------
+// cell-count prefix sums (tile-major cell order)
+// ---------------------------------------------------------------------------------------------
+
+struct ScanSp {
+    int* cnt_acc;
+    int* stay_acc;
+    int* seg;
+    int* start;
+    int* stay;
+    int* arr;
+    int* tile_total;
+    int* tile_base;
+};
+struct ScanArgs {
+    Geo g;
+    ScanSp sp[IPIC_MAX_SPECIES];
+};
+
+__global__ __launch_bounds__(NT) void k_scan_tiles(ScanArgs A) {
+    const Geo& g = A.g;
+    const ScanSp& S = A.sp[blockIdx.y];
+    __shared__ int s_w[NT / 64];
+    const int t = blockIdx.x;
+    int v = 0;
+    for (int i = threadIdx.x; i < g.TC; i += NT) v += S.cnt_acc[t * g.TC + i];
+    int tot;
+    block_exscan(v, s_w, tot);
+    if (threadIdx.x == 0) S.tile_total[t] = tot;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_top(ScanArgs A) {
+    const Geo& g = A.g;
+    const ScanSp& S = A.sp[blockIdx.x];
+    __shared__ int s_part[1024];
+    const int per = (g.ntiles + 1023) / 1024;
+    const int b0 = threadIdx.x * per;
+    int v = 0;
+    for (int k = 0; k < per; ++k)
+        if (b0 + k < g.ntiles) v += S.tile_total[b0 + k];
+    s_part[threadIdx.x] = v;
+    __syncthreads();
+    // Hillis-Steele inclusive scan over 1024 partial sums
+    for (int d = 1; d < 1024; d <<= 1) {
+        const int y = threadIdx.x >= d ? s_part[threadIdx.x - d] : 0;
+        __syncthreads();
+        s_part[threadIdx.x] += y;
+        __syncthreads();
+    }
+    int run = s_part[threadIdx.x] - v;
+    for (int k = 0; k < per; ++k)
+        if (b0 + k < g.ntiles) {
+            S.tile_base[b0 + k] = run;
+            run += S.tile_total[b0 + k];
+        }
+    if (threadIdx.x == 1023) S.tile_base[g.ntiles] = s_part[1023];
+}
+
+// New starts = tile base + in-tile exclusive prefix of the counts; the previous starts
+// become the segments of the array just written; stayers move to `stay`; accumulators and
+// arrival counters are cleared for the next pass.
+__global__ __launch_bounds__(NT) void k_scan_cells(ScanArgs A) {
+    const Geo& g = A.g;
+    const ScanSp& S = A.sp[blockIdx.y];
+    __shared__ int s_w[NT / 64];
+    const int t = blockIdx.x;
+    const int c0 = 2 * threadIdx.x, c1 = c0 + 1;
+    const int id0 = t * g.TC + c0, id1 = t * g.TC + c1;
+    int v0 = 0, v1 = 0;
+    if (c0 < g.TC) v0 = S.cnt_acc[id0];
+    if (c1 < g.TC) v1 = S.cnt_acc[id1];
+    int tot;
+    const int ex = block_exscan(v0 + v1, s_w, tot) + S.tile_base[t];
+    if (c0 < g.TC) {
+        S.seg[id0] = S.start[id0];
+        S.start[id0] = ex;
+        S.stay[id0] = S.stay_acc[id0];
+        S.stay_acc[id0] = 0;
+        S.cnt_acc[id0] = 0;
+        S.arr[id0] = 0;
+    }
+    if (c1 < g.TC) {
+        S.seg[id1] = S.start[id1];
+        S.start[id1] = ex + v0;
+        S.stay[id1] = S.stay_acc[id1];
+        S.stay_acc[id1] = 0;
+        S.cnt_acc[id1] = 0;
+        S.arr[id1] = 0;
+    }
+    if (t == g.ntiles - 1 && threadIdx.x == 0) {
+        S.seg[g.ncell] = S.start[g.ncell];
+        S.start[g.ncell] = S.tile_base[g.ntiles];
+    }
+}
+
+// After a REBIN pass the array is sorted by current cell: segments = starts, all stay.
+__global__ __launch_bounds__(256) void k_canon_fix(ScanArgs A) {
+    const Geo& g = A.g;
+    const ScanSp& S = A.sp[blockIdx.y];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > g.ncell) return;
+    S.seg[i] = S.start[i];
+    if (i < g.ncell) {
+        S.stay[i] = S.start[i + 1] - S.start[i];
+        S.arr[i] = 0;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// synthetic initialisation and FP64 reductions
+// ---------------------------------------------------------------------------------------------
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+__global__ __launch_bounds__(256) void k_init(Geo g, const int* start, float* X, uint32_t* ids, long long cap,
+                                              long long total, uint64_t seed, int species, double vx_, double vy_,
+                                              double vz_, double dx_, double dy_, double dz_) {
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= total) return;
+    int lo = 0, hi = g.ncell - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (start[mid] <= p) lo = mid; else hi = mid - 1;
+    }
+    const int cid = lo;
+    const long long idx = p - start[cid];
+    const int t = cid / g.TC, l = cid % g.TC;
+    const int tx = t / (g.NTl[1] * g.NTl[2]), ty = (t / g.NTl[2]) % g.NTl[1], tz = t % g.NTl[2];
+    const int x = tx * g.T[0] + l / (g.T[1] * g.T[2]);
+    const int y = ty * g.T[1] + (l / g.T[2]) % g.T[1];
+    const int z = tz * g.T[2] + l % g.T[2];
+    const long long row = ((long long)x * g.n[1] + y) * g.n[2] + z;
+    uint64_t key = mix64(mix64(mix64(seed) ^ (uint64_t)(uint32_t)species) ^ (uint64_t)row);
+    key = mix64(key ^ (uint64_t)idx);
+    double r[7];
+    for (int k = 0; k < 7; ++k)
+        r[k] = (double)(mix64(key + (uint64_t)(k + 1) * 0xD1B54A32D192ED03ULL) >> 11) * (1.0 / 9007199254740992.0);
+    const double twopi = 6.283185307179586;
+    const double m0 = sqrt(-2.0 * log(1.0 - r[3]));
+    const double m1 = sqrt(-2.0 * log(1.0 - r[5]));
+    const double n0 = m0 * cos(twopi * r[4]), n1 = m0 * sin(twopi * r[4]), n2 = m1 * cos(twopi * r[6]);
+    X[p] = (float)r[0];
+    X[cap + p] = (float)r[1];
+    X[2 * cap + p] = (float)r[2];
+    X[3 * cap + p] = (float)(dx_ + vx_ * n0);
+    X[4 * cap + p] = (float)(dy_ + vy_ * n1);
+    X[5 * cap + p] = (float)(dz_ + vz_ * n2);
+    if (ids) ids[p] = (uint32_t)p;
+}
+
+__device__ __forceinline__ double block_sum_d(double v, double* s) {
+    for (int d = 32; d > 0; d >>= 1) v += __shfl_down(v, d, 64);
+    if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double r = 0;
+    if (threadIdx.x == 0)
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) r += s[w];
+    __syncthreads();
+    return r;
+}
+
+__device__ __forceinline__ double block_max_d(double v, double* s) {
+    for (int d = 32; d > 0; d >>= 1) v = fmax(v, __shfl_down(v, d, 64));
+    if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double r = 0;
+    if (threadIdx.x == 0)
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) r = fmax(r, s[w]);
+    __syncthreads();
+    return r;
+}
+
+// FP64 per-block partials of sum 0.5 m |v|^2, sum m |v|, count (cell.h:1315-1338).
+__global__ __launch_bounds__(256) void k_reduce_particles(const float* X, long long cap, long long nslots,
+                                                          double mass, double* out) {
+    __shared__ double s[4];
+    double ke = 0, mo = 0, cnt = 0;
+    for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < nslots;
+         p += (long long)gridDim.x * blockDim.x) {
+        const float ox = X[p];
+        if (!(ox == ox)) continue;
+        const double vx = X[3 * cap + p], vy = X[4 * cap + p], vz = X[5 * cap + p];
+        const double v2 = vx * vx + vy * vy + vz * vz;
+        ke += 0.5 * mass * v2;
+        mo += mass * sqrt(v2);
+        cnt += 1.0;
+    }
+    ke = block_sum_d(ke, s);
+    mo = block_sum_d(mo, s);
+    cnt = block_sum_d(cnt, s);
+    if (threadIdx.x == 0) {
+        out[3 * blockIdx.x + 0] = ke;
+        out[3 * blockIdx.x + 1] = mo;
+        out[3 * blockIdx.x + 2] = cnt;
+    }
+}
+
+__device__ __forceinline__ void tot_field(const float4* d, const float4* bg, const float c0[3], int i, double r[3]) {
+    const float4 v = d[i];
+    r[0] = (double)c0[0] + v.x;
+    r[1] = (double)c0[1] + v.y;
+    r[2] = (double)c0[2] + v.z;
+    if (bg) {
+        const float4 u = bg[i];
+        r[0] += u.x; r[1] += u.y; r[2] += u.z;
+    }
+}
+
+// FP64 per-block partials of sum|E|^2, sum|Bc|^2, sum E.J, sum (div+ Bc)^2, max |div+ Bc|
+// of the total field (background + perturbation).
+__global__ __launch_bounds__(256) void k_reduce_fields(FieldArgs F, double* out) {
+    const Geo& g = F.g;
+    __shared__ double s[4];
+    double se = 0, sb = 0, sej = 0, sd2 = 0, md = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < g.ncell; i += gridDim.x * blockDim.x) {
+        int x, y, z;
+        xyz_of(g, i, x, y, z);
+        const float4 j = F.J[i];
+        double e[3], b[3];
+        tot_field(F.E, F.Ebg, g.E0, i, e);
+        tot_field(F.Bc, F.Bcbg, g.B0, i, b);
+        se += e[0] * e[0] + e[1] * e[1] + e[2] * e[2];
+        sb += b[0] * b[0] + b[1] * b[1] + b[2] * b[2];
+        sej += e[0] * j.x + e[1] * j.y + e[2] * j.z;
+        double bpx[3], bpy[3], bpz[3];
+        tot_field(F.Bc, F.Bcbg, g.B0, nid_of(g, wrapc(x + 1, g.n[0]), y, z), bpx);
+        tot_field(F.Bc, F.Bcbg, g.B0, nid_of(g, x, wrapc(y + 1, g.n[1]), z), bpy);
+        tot_field(F.Bc, F.Bcbg, g.B0, nid_of(g, x, y, wrapc(z + 1, g.n[2])), bpz);
+        const double dv = (bpx[0] - b[0]) * g.inv_w[0] + (bpy[1] - b[1]) * g.inv_w[1] +
+                          (bpz[2] - b[2]) * g.inv_w[2];
+        sd2 += dv * dv;
+        md = fmax(md, fabs(dv));
+    }
+    se = block_sum_d(se, s);
+    sb = block_sum_d(sb, s);
+    sej = block_sum_d(sej, s);
+    sd2 = block_sum_d(sd2, s);
+    md = block_max_d(md, s);
+    if (threadIdx.x == 0) {
+        double* o = out + 5 * blockIdx.x;
+        o[0] = se; o[1] = sb; o[2] = sej; o[3] = sd2; o[4] = md;
+    }
+}
+
+__global__ void k_fill_f4(float4* p, long long n, float4 v) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+}  // namespace
+
+// =============================================================================================
+// host side
+// =============================================================================================
+
+namespace {
+
+thread_local std::string g_last_error;
+
+struct Species {
+    double q = 0, qom = 0;
+    float* X[2] = {nullptr, nullptr};
+    uint32_t* ids[2] = {nullptr, nullptr};
+    long long cap = 0;
+    int cur = 0;
+    int* seg = nullptr;
+    int* start = nullptr;
+    int* stay = nullptr;
+    int* cnt_acc = nullptr;
+    int* stay_acc = nullptr;
+    int* arr = nullptr;
+    int* tile_total = nullptr;
+    int* tile_base = nullptr;
+};
+
+}  // namespace
+
+struct ipic_ctx {
+    ipic_desc d{};
+    Geo g{};
+    hipStream_t stream = nullptr;
+    float4* E = nullptr;
+    float4* Bc = nullptr;
+    float4* Bn = nullptr;
+    float4* J = nullptr;
+    float4* Eg = nullptr;     // optional device arrays of the background split
+    float4* Ebg = nullptr;
+    float4* Bcbg = nullptr;
+    float4* Bnbg = nullptr;
+    float4* SE = nullptr;
+    float4* SB = nullptr;
+    std::vector<double> bgE, bgBc;  // background (state at the last field upload), FP64
+    float* jpart = nullptr;
+    float* jover = nullptr;
+    int* jflag = nullptr;
+    int* jpairs = nullptr;
+    unsigned long long* removed = nullptr;
+    int* derr = nullptr;
+    double* red = nullptr;
+    int red_blocks = 1024;
+    Species sp[IPIC_MAX_SPECIES];
+    bool j_valid = false;
+    bool canonical = true;
+    bool track = false;
+    std::vector<void*> allocs;
+    std::string err;
+    hipEvent_t ev[8] = {};
+};
+
+namespace {
+
+int fail(ipic_ctx* ctx, int code, const std::string& msg) {
+    if (ctx) ctx->err = msg;
+    g_last_error = msg;
+    return code;
+}
+
+#define HIPCK(ctx, expr)                                                                        \
+    do {                                                                                        \
+        hipError_t e_ = (expr);                                                                 \
+        if (e_ != hipSuccess)                                                                   \
+            return fail((ctx), IPIC_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+template <class T>
+int dalloc(ipic_ctx* ctx, T** p, size_t count) {
+    void* q = nullptr;
+    const size_t bytes = std::max<size_t>(count * sizeof(T), 256);
+    HIPCK(ctx, hipMalloc(&q, bytes));
+    HIPCK(ctx, hipMemsetAsync(q, 0, bytes, ctx->stream));
+    ctx->allocs.push_back(q);
+    *p = (T*)q;
+    return 0;
+}
+
+void dfree(ipic_ctx* ctx, void* p) {
+    if (!p) return;
+    auto it = std::find(ctx->allocs.begin(), ctx->allocs.end(), p);
+    if (it != ctx->allocs.end()) ctx->allocs.erase(it);
+    (void)hipFree(p);
+}
+
+int largest_divisor_le(int n, int cap) {
+    for (int t = std::min(n, cap); t >= 1; --t)
+        if (n % t == 0) return t;
+    return 1;
+}
+
+// per axis, the (tile, local node) pairs whose J partial region [tile*T - 1, tile*T + T + 2)
+// contains coordinate x (periodic)
+int build_jpairs(const Geo& g, std::vector<int>& tab) {
+    tab.assign((size_t)3 * g.nmax * (1 + JPMAX), 0);
+    for (int d = 0; d < 3; ++d)
+        for (int x = 0; x < g.n[d]; ++x) {
+            int* e = &tab[((size_t)d * g.nmax + x) * (1 + JPMAX)];
+            for (int l = 0; l < g.nj[d]; ++l) {
+                int s = (x + 1 - l) % g.n[d];
+                if (s < 0) s += g.n[d];
+                if (s % g.T[d] == 0) {
+                    if (e[0] >= JPMAX) return -1;
+                    e[1 + e[0]] = ((s / g.T[d]) << 8) | l;
+                    e[0]++;
+                }
+            }
+        }
+    return 0;
+}
+
+PushArgs push_args(ipic_ctx* ctx) {
+    PushArgs A{};
+    A.g = ctx->g;
+    A.nsp = ctx->d.nspecies;
+    const double vol = ctx->d.cell_width[0] * ctx->d.cell_width[1] * ctx->d.cell_width[2];
+    for (int s = 0; s < A.nsp; ++s) {
+        Species& S = ctx->sp[s];
+        SpArgs& a = A.sp[s];
+        a.in = S.X[S.cur];
+        a.out = S.X[1 - S.cur];
+        a.id_in = S.ids[S.cur];
+        a.id_out = S.ids[1 - S.cur];
+        a.cap = S.cap;
+        a.seg = S.seg;
+        a.start = S.start;
+        a.stay = S.stay;
+        a.arr = S.arr;
+        a.cnt_acc = S.cnt_acc;
+        a.stay_acc = S.stay_acc;
+        a.qom = (float)S.qom;
+        a.qs = (float)(S.q / (8.0 * vol));
+    }
+    A.E = ctx->Eg ? ctx->Eg : ctx->E;
+    A.Bn = ctx->Bn;
+    A.jpart = ctx->jpart;
+    A.jover = ctx->jover;
+    A.jflag = ctx->jflag;
+    A.removed = ctx->removed;
+    A.err = ctx->derr;
+    return A;
+}
+
+FieldArgs field_args(ipic_ctx* ctx) {
+    FieldArgs F{};
+    F.g = ctx->g;
+    F.E = ctx->E;
+    F.Bc = ctx->Bc;
+    F.Bn = ctx->Bn;
+    F.J = ctx->J;
+    F.Eg = ctx->Eg;
+    F.Ebg = ctx->Ebg;
+    F.Bcbg = ctx->Bcbg;
+    F.Bnbg = ctx->Bnbg;
+    F.SE = ctx->SE;
+    F.SB = ctx->SB;
+    F.jpart = ctx->jpart;
+    F.jover = ctx->jover;
+    F.jflag = ctx->jflag;
+    F.jpairs = ctx->jpairs;
+    F.dt = (float)(ctx->d.dt / ctx->d.field_substeps);
+    F.c2 = (float)(ctx->d.c * ctx->d.c);
+    return F;
+}
+
+ScanArgs scan_args(ipic_ctx* ctx) {
+    ScanArgs A{};
+    A.g = ctx->g;
+    for (int s = 0; s < ctx->d.nspecies; ++s) {
+        Species& S = ctx->sp[s];
+        A.sp[s] = ScanSp{S.cnt_acc, S.stay_acc, S.seg, S.start, S.stay, S.arr, S.tile_total, S.tile_base};
+    }
+    return A;
+}
+
+int launch_particles(ipic_ctx* ctx, int mode) {
+    PushArgs A = push_args(ctx);
+    dim3 grid(ctx->g.ntiles), block(NT);
+    if (ctx->track) {
+        if (mode == MODE_PUSH) hipLaunchKernelGGL((k_particles<MODE_PUSH, true>), grid, block, 0, ctx->stream, A);
+        else if (mode == MODE_DEPOSIT) hipLaunchKernelGGL((k_particles<MODE_DEPOSIT, true>), grid, block, 0, ctx->stream, A);
+        else hipLaunchKernelGGL((k_particles<MODE_REBIN, true>), grid, block, 0, ctx->stream, A);
+    } else {
+        if (mode == MODE_PUSH) hipLaunchKernelGGL((k_particles<MODE_PUSH, false>), grid, block, 0, ctx->stream, A);
+        else if (mode == MODE_DEPOSIT) hipLaunchKernelGGL((k_particles<MODE_DEPOSIT, false>), grid, block, 0, ctx->stream, A);
+        else hipLaunchKernelGGL((k_particles<MODE_REBIN, false>), grid, block, 0, ctx->stream, A);
+    }
+    HIPCK(ctx, hipGetLastError());
+    if (mode != MODE_DEPOSIT)
+        for (int s = 0; s < ctx->d.nspecies; ++s) ctx->sp[s].cur = 1 - ctx->sp[s].cur;
+    return 0;
+}
+
+int launch_scan(ipic_ctx* ctx) {
+    ScanArgs A = scan_args(ctx);
+    const int ns = ctx->d.nspecies;
+    hipLaunchKernelGGL(k_scan_tiles, dim3(ctx->g.ntiles, ns), dim3(NT), 0, ctx->stream, A);
+    hipLaunchKernelGGL(k_scan_top, dim3(ns), dim3(1024), 0, ctx->stream, A);
+    hipLaunchKernelGGL(k_scan_cells, dim3(ctx->g.ntiles, ns), dim3(NT), 0, ctx->stream, A);
+    HIPCK(ctx, hipGetLastError());
+    return 0;
+}
+
+int launch_field(ipic_ctx* ctx) {
+    FieldArgs F = field_args(ctx);
+    const int nb = (ctx->g.ncell + 255) / 256;
+    for (int s = 0; s < ctx->d.field_substeps; ++s) {
+        hipLaunchKernelGGL(k_field_E, dim3(nb), dim3(256), 0, ctx->stream, F, s == 0 ? 1 : 0);
+        hipLaunchKernelGGL(k_field_B, dim3(nb), dim3(256), 0, ctx->stream, F);
+    }
+    hipLaunchKernelGGL(k_c2n, dim3(nb), dim3(256), 0, ctx->stream, F);
+    HIPCK(ctx, hipGetLastError());
+    return 0;
+}
+
+int check_device_flag(ipic_ctx* ctx);
+
+int canonicalize(ipic_ctx* ctx) {
+    if (ctx->canonical) return 0;
+    int rc = launch_particles(ctx, MODE_REBIN);
+    if (rc) return rc;
+    ScanArgs A = scan_args(ctx);
+    hipLaunchKernelGGL(k_canon_fix, dim3((ctx->g.ncell + 1 + 255) / 256, ctx->d.nspecies), dim3(256), 0,
+                       ctx->stream, A);
+    HIPCK(ctx, hipGetLastError());
+    ctx->canonical = true;
+    return check_device_flag(ctx);
+}
+
+// live particles of species s (the layout the next pass writes)
+int read_total(ipic_ctx* ctx, int s, long long* total) {
+    int v = 0;
+    HIPCK(ctx, hipMemcpyAsync(&v, ctx->sp[s].start + ctx->g.ncell, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCK(ctx, hipStreamSynchronize(ctx->stream));
+    *total = v;
+    return 0;
+}
+
+// slots of the current particle array (its segments), including slots of particles the
+// sink removed during the pass that wrote it
+int read_slots(ipic_ctx* ctx, int s, long long* slots) {
+    int v = 0;
+    HIPCK(ctx, hipMemcpyAsync(&v, ctx->sp[s].seg + ctx->g.ncell, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCK(ctx, hipStreamSynchronize(ctx->stream));
+    *slots = v;
+    return 0;
+}
+
+// cell coordinates of a tile-major cell id
+inline void cid_xyz(const Geo& g, int cid, int& x, int& y, int& z) {
+    const int t = cid / g.TC, l = cid % g.TC;
+    const int tx = t / (g.NTl[1] * g.NTl[2]), ty = (t / g.NTl[2]) % g.NTl[1], tz = t % g.NTl[2];
+    x = tx * g.T[0] + l / (g.T[1] * g.T[2]);
+    y = ty * g.T[1] + (l / g.T[2]) % g.T[1];
+    z = tz * g.T[2] + l % g.T[2];
+}
+
+inline int host_cid(const Geo& g, int x, int y, int z) {
+    const int tx = x / g.T[0], ty = y / g.T[1], tz = z / g.T[2];
+    const int lx = x - tx * g.T[0], ly = y - ty * g.T[1], lz = z - tz * g.T[2];
+    return ((tx * g.NTl[1] + ty) * g.NTl[2] + tz) * g.TC + (lx * g.T[1] + ly) * g.T[2] + lz;
+}
+
+int alloc_species_buffers(ipic_ctx* ctx, int s, long long cap) {
+    Species& S = ctx->sp[s];
+    if (cap > S.cap) {
+        for (int b = 0; b < 2; ++b) {
+            dfree(ctx, S.X[b]);
+            dfree(ctx, S.ids[b]);
+            S.X[b] = nullptr;
+            S.ids[b] = nullptr;
+        }
+        const long long c = std::max<long long>(cap, 1);
+        for (int b = 0; b < 2; ++b) {
+            if (int rc = dalloc(ctx, &S.X[b], (size_t)6 * c)) return rc;
+            if (ctx->track)
+                if (int rc = dalloc(ctx, &S.ids[b], (size_t)c)) return rc;
+        }
+        S.cap = c;
+    }
+    S.cur = 0;
+    return 0;
+}
+
+// set the per-cell bookkeeping of a freshly sorted (canonical) array from host starts
+int set_layout(ipic_ctx* ctx, int s, const std::vector<int>& start) {
+    Species& S = ctx->sp[s];
+    const int nc = ctx->g.ncell;
+    std::vector<int> stay(nc);
+    for (int c = 0; c < nc; ++c) stay[c] = start[c + 1] - start[c];
+    HIPCK(ctx, hipMemcpyAsync(S.seg, start.data(), (nc + 1) * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+    HIPCK(ctx, hipMemcpyAsync(S.start, start.data(), (nc + 1) * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+    HIPCK(ctx, hipMemcpyAsync(S.stay, stay.data(), nc * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+    HIPCK(ctx, hipMemsetAsync(S.cnt_acc, 0, nc * sizeof(int), ctx->stream));
+    HIPCK(ctx, hipMemsetAsync(S.stay_acc, 0, nc * sizeof(int), ctx->stream));
+    HIPCK(ctx, hipMemsetAsync(S.arr, 0, nc * sizeof(int), ctx->stream));
+    HIPCK(ctx, hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+// total E (nodes) and Bc (centres) in FP64: background + device perturbation
+int total_fields(ipic_ctx* ctx, double* E, double* Bc) {
+    const int nc = ctx->g.ncell;
+    std::vector<float4> h(nc);
+    HIPCK(ctx, hipStreamSynchronize(ctx->stream));
+    for (int f = 0; f < 2; ++f) {
+        double* dst = f == 0 ? E : Bc;
+        if (!dst) continue;
+        const std::vector<double>& bg = f == 0 ? ctx->bgE : ctx->bgBc;
+        HIPCK(ctx, hipMemcpy(h.data(), f == 0 ? ctx->E : ctx->Bc, nc * sizeof(float4), hipMemcpyDeviceToHost));
+        for (int i = 0; i < nc; ++i) {
+            const double b0 = bg.empty() ? 0.0 : bg[3 * i], b1 = bg.empty() ? 0.0 : bg[3 * i + 1],
+                         b2 = bg.empty() ? 0.0 : bg[3 * i + 2];
+            dst[3 * i] = b0 + h[i].x;
+            dst[3 * i + 1] = b1 + h[i].y;
+            dst[3 * i + 2] = b2 + h[i].z;
+        }
+    }
+    return 0;
+}
+
+// reads and clears the device consistency flag set by the particle kernel guards
+int check_device_flag(ipic_ctx* ctx) {
+    int v = 0;
+    HIPCK(ctx, hipMemcpyAsync(&v, ctx->derr, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCK(ctx, hipStreamSynchronize(ctx->stream));
+    if (v) {
+        const int z = 0;
+        HIPCK(ctx, hipMemcpy(ctx->derr, &z, sizeof(int), hipMemcpyHostToDevice));
+        return fail(ctx, IPIC_E_STATE,
+                    std::string("particle layout guard fired (code ") + std::to_string(v) +
+                        "): per-cell counts inconsistent; state is invalid");
+    }
+    return 0;
+}
+
+int check_ctx(ipic_ctx* ctx) {
+    if (!ctx) return fail(nullptr, IPIC_E_ARG, "null context");
+    return 0;
+}
+
+}  // namespace
+
+// =============================================================================================
+// C ABI
+// =============================================================================================
+
+extern "C" {
+
+const char* ipic_gpu_last_error(const ipic_ctx* ctx) {
+    if (ctx && !ctx->err.empty()) return ctx->err.c_str();
+    return g_last_error.c_str();
+}
+
+int ipic_gpu_device_info(char* name, int32_t name_len, int32_t* cu_count, int64_t* hbm_bytes) {
+    int dev = 0;
+    hipDeviceProp_t p;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipGetDeviceProperties(&p, dev);
+    if (e != hipSuccess) return fail(nullptr, IPIC_E_HIP, std::string("hipGetDeviceProperties: ") + hipGetErrorString(e));
+    if (name && name_len > 0) {
+        std::snprintf(name, (size_t)name_len, "%s (%s)", p.name, p.gcnArchName);
+    }
+    if (cu_count) *cu_count = p.multiProcessorCount;
+    if (hbm_bytes) *hbm_bytes = (int64_t)p.totalGlobalMem;
+    return 0;
+}
+
+int ipic_gpu_create(const ipic_desc* desc, ipic_ctx** out) {
+    if (!desc || !out) return fail(nullptr, IPIC_E_ARG, "null argument");
+    *out = nullptr;
+    const ipic_desc& d = *desc;
+    for (int k = 0; k < 3; ++k) {
+        if (d.n[k] <= 0) return fail(nullptr, IPIC_E_ARG, "grid size must be positive");
+        if (!(d.cell_width[k] > 0)) return fail(nullptr, IPIC_E_ARG, "cell width must be positive");
+    }
+    if (!(d.dt > 0) || !(d.c > 0)) return fail(nullptr, IPIC_E_ARG, "dt and c must be positive");
+    if (d.nspecies < 1 || d.nspecies > IPIC_MAX_SPECIES) return fail(nullptr, IPIC_E_ARG, "nspecies out of range");
+    const long long ncell = (long long)d.n[0] * d.n[1] * d.n[2];
+    if (ncell > (1LL << 30)) return fail(nullptr, IPIC_E_RANGE, "grid too large");
+
+    ipic_ctx* ctx = new ipic_ctx();
+    ctx->d = d;
+    if (ctx->d.field_substeps < 1) ctx->d.field_substeps = 1;
+    ctx->track = (d.flags & IPIC_FLAG_TRACK_IDS) != 0;
+    Geo& g = ctx->g;
+    for (int k = 0; k < 3; ++k) {
+        g.n[k] = d.n[k];
+        int T = d.tile[k] > 0 ? d.tile[k] : largest_divisor_le(d.n[k], 8);
+        if (T > 8 || d.n[k] % T != 0) {
+            delete ctx;
+            return fail(nullptr, IPIC_E_ARG, "tile size must divide the grid and be at most 8");
+        }
+        g.T[k] = T;
+        g.NTl[k] = d.n[k] / T;
+        g.nj[k] = T + 3;
+        g.w[k] = (float)d.cell_width[k];
+        g.inv_w[k] = (float)(1.0 / d.cell_width[k]);
+        g.origin[k] = (float)d.origin[k];
+        g.sink_c[k] = (float)d.sink_center[k];
+    }
+    g.TC = g.T[0] * g.T[1] * g.T[2];
+    g.ntiles = g.NTl[0] * g.NTl[1] * g.NTl[2];
+    g.ncell = (int)ncell;
+    g.NJ = g.nj[0] * g.nj[1] * g.nj[2];
+    g.nmax = std::max(g.n[0], std::max(g.n[1], g.n[2]));
+    g.dt = (float)d.dt;
+    g.inv_vol = (float)(1.0 / (d.cell_width[0] * d.cell_width[1] * d.cell_width[2]));
+    g.pi_c = (float)(M_PI * d.c);
+    g.sink = d.sink ? 1 : 0;
+    g.sink_r2 = (float)(d.sink_radius * d.sink_radius);
+
+    std::vector<int> jp;
+    if (build_jpairs(g, jp) != 0) {
+        delete ctx;
+        return fail(nullptr, IPIC_E_ARG, "tile/grid combination not supported");
+    }
+    hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete ctx;
+        return fail(nullptr, IPIC_E_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    }
+    int rc = 0;
+    const size_t nc = (size_t)g.ncell;
+    rc = rc ? rc : dalloc(ctx, &ctx->E, nc);
+    rc = rc ? rc : dalloc(ctx, &ctx->Bc, nc);
+    rc = rc ? rc : dalloc(ctx, &ctx->Bn, nc);
+    rc = rc ? rc : dalloc(ctx, &ctx->J, nc);
+    rc = rc ? rc : dalloc(ctx, &ctx->jpart, (size_t)g.ntiles * 3 * g.NJ);
+    rc = rc ? rc : dalloc(ctx, &ctx->jover, nc * 3);
+    rc = rc ? rc : dalloc(ctx, &ctx->jflag, 1);
+    rc = rc ? rc : dalloc(ctx, &ctx->jpairs, jp.size());
+    rc = rc ? rc : dalloc(ctx, &ctx->removed, 1);
+    rc = rc ? rc : dalloc(ctx, &ctx->derr, 1);
+    rc = rc ? rc : dalloc(ctx, &ctx->red, (size_t)ctx->red_blocks * 5);
+    for (int s = 0; s < d.nspecies && !rc; ++s) {
+        Species& S = ctx->sp[s];
+        S.q = d.species[s].q;
+        S.qom = d.species[s].qom;
+        rc = rc ? rc : dalloc(ctx, &S.seg, nc + 1);
+        rc = rc ? rc : dalloc(ctx, &S.start, nc + 1);
+        rc = rc ? rc : dalloc(ctx, &S.stay, nc);
+        rc = rc ? rc : dalloc(ctx, &S.cnt_acc, nc);
+        rc = rc ? rc : dalloc(ctx, &S.stay_acc, nc);
+        rc = rc ? rc : dalloc(ctx, &S.arr, nc);
+        rc = rc ? rc : dalloc(ctx, &S.tile_total, (size_t)g.ntiles);
+        rc = rc ? rc : dalloc(ctx, &S.tile_base, (size_t)g.ntiles + 1);
+    }
+    if (!rc) {
+        e = hipMemcpyAsync(ctx->jpairs, jp.data(), jp.size() * sizeof(int), hipMemcpyHostToDevice, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) rc = fail(ctx, IPIC_E_HIP, std::string("upload: ") + hipGetErrorString(e));
+    }
+    if (!rc)
+        for (int k = 0; k < 8; ++k)
+            if (hipEventCreate(&ctx->ev[k]) != hipSuccess) rc = fail(ctx, IPIC_E_HIP, "hipEventCreate");
+    if (rc) {
+        std::string msg = ctx->err;
+        ipic_gpu_destroy(ctx);
+        g_last_error = msg;
+        return rc;
+    }
+    *out = ctx;
+    return 0;
+}
+
+int ipic_gpu_destroy(ipic_ctx* ctx) {
+    if (!ctx) return 0;
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (void* p : ctx->allocs) (void)hipFree(p);
+    for (int k = 0; k < 8; ++k)
+        if (ctx->ev[k]) (void)hipEventDestroy(ctx->ev[k]);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return 0;
+}
+
+int ipic_gpu_upload_particles(ipic_ctx* ctx, int32_t species, int64_t n, const double* pos, const double* vel) {
+    if (int rc = check_ctx(ctx)) return rc;
+    if (species < 0 || species >= ctx->d.nspecies) return fail(ctx, IPIC_E_ARG, "species out of range");
+    if (n < 0 || (n > 0 && (!pos || !vel))) return fail(ctx, IPIC_E_ARG, "invalid particle arrays");
+    if (n >= (1LL << 31)) return fail(ctx, IPIC_E_RANGE, "at most 2^31-1 particles per species");
+    if (int rc = canonicalize(ctx)) return rc;
+    const Geo& g = ctx->g;
+    const int nc = g.ncell;
+    std::vector<int> cell((size_t)n);
+    std::vector<float> off((size_t)n * 3);
+    std::vector<int> count(nc + 1, 0);
+    for (int64_t i = 0; i < n; ++i) {
+        int ci[3];
+        for (int k = 0; k < 3; ++k) {
+            const double L = (double)g.n[k] * ctx->d.cell_width[k];
+            double x = pos[3 * i + k] - ctx->d.origin[k];
+            if (!(std::isfinite(x))) return fail(ctx, IPIC_E_ARG, "non-finite particle position");
+            x = std::fmod(x, L);
+            if (x < 0) x += L;
+            double qd = x / ctx->d.cell_width[k];
+            long long cc = (long long)std::floor(qd);
+            if (cc >= g.n[k]) cc = g.n[k] - 1;
+            if (cc < 0) cc = 0;
+            ci[k] = (int)cc;
+            float o = (float)(qd - (double)cc);
+            if (o < 0.0f) o = 0.0f;
+            if (o > 1.0f) o = 1.0f;
+            off[3 * i + k] = o;
+        }
+        cell[i] = host_cid(g, ci[0], ci[1], ci[2]);
+        count[cell[i] + 1]++;
+    }
+    std::vector<int> start(nc + 1, 0);
+    for (int c = 0; c < nc; ++c) start[c + 1] = start[c] + count[c + 1];
+    std::vector<int> fillp(start.begin(), start.end() - 1);
+    const long long cap = std::max<long long>(n, 1);
+    std::vector<float> X((size_t)6 * cap, 0.0f);
+    std::vector<uint32_t> ids(ctx->track ? cap : 0);
+    for (int64_t i = 0; i < n; ++i) {
+        const int p = fillp[cell[i]]++;
+        X[p] = off[3 * i];
+        X[cap + p] = off[3 * i + 1];
+        X[2 * cap + p] = off[3 * i + 2];
+        X[3 * cap + p] = (float)vel[3 * i];
+        X[4 * cap + p] = (float)vel[3 * i + 1];
+        X[5 * cap + p] = (float)vel[3 * i + 2];
+        if (ctx->track) ids[p] = (uint32_t)i;
+    }
+    if (int rc = alloc_species_buffers(ctx, species, cap)) return rc;
+    Species& S = ctx->sp[species];
+    if (S.cap != cap) {
+        // keep the array stride of the allocation
+        std::vector<float> Y((size_t)6 * S.cap, 0.0f);
+        for (int k = 0; k < 6; ++k) std::memcpy(&Y[(size_t)k * S.cap], &X[(size_t)k * cap], (size_t)n * sizeof(float));
+        X.swap(Y);
+    }
+    HIPCK(ctx, hipMemcpyAsync(S.X[S.cur], X.data(), X.size() * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+    if (ctx->track && n > 0)
+        HIPCK(ctx, hipMemcpyAsync(S.ids[S.cur], ids.data(), (size_t)n * sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream));
+    if (int rc = set_layout(ctx, species, start)) return rc;
+    ctx->j_valid = false;
+    return 0;
+}
+
+int ipic_gpu_init_particles(ipic_ctx* ctx, int32_t species, const int32_t* ppc, int32_t uniform_ppc,
+                            const double vth[3], const double drift[3], uint64_t seed) {
+    if (int rc = check_ctx(ctx)) return rc;
+    if (species < 0 || species >= ctx->d.nspecies) return fail(ctx, IPIC_E_ARG, "species out of range");
+    if (!vth || !drift) return fail(ctx, IPIC_E_ARG, "null vth/drift");
+    if (!ppc && uniform_ppc < 0) return fail(ctx, IPIC_E_ARG, "negative particles per cell");
+    if (int rc = canonicalize(ctx)) return rc;
+    const Geo& g = ctx->g;
+    const int nc = g.ncell;
+    std::vector<int> start(nc + 1, 0);
+    long long total = 0;
+    for (int cid = 0; cid < nc; ++cid) {
+        int x, y, z;
+        cid_xyz(g, cid, x, y, z);
+        const int row = (x * g.n[1] + y) * g.n[2] + z;
+        const long long k = ppc ? ppc[row] : uniform_ppc;
+        if (k < 0) return fail(ctx, IPIC_E_ARG, "negative particles per cell");
+        total += k;
+        if (total >= (1LL << 31)) return fail(ctx, IPIC_E_RANGE, "at most 2^31-1 particles per species");
+        start[cid + 1] = (int)total;
+    }
+    if (int rc = alloc_species_buffers(ctx, species, std::max<long long>(total, 1))) return rc;
+    Species& S = ctx->sp[species];
+    if (int rc = set_layout(ctx, species, start)) return rc;
+    if (total > 0) {
+        const unsigned nb = (unsigned)((total + 255) / 256);
+        hipLaunchKernelGGL(k_init, dim3(nb), dim3(256), 0, ctx->stream, g, S.start, S.X[S.cur],
+                           ctx->track ? S.ids[S.cur] : nullptr, S.cap, total, seed, (int)species, vth[0], vth[1], vth[2],
+                           drift[0], drift[1], drift[2]);
+        HIPCK(ctx, hipGetLastError());
+    }
+    HIPCK(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->j_valid = false;
+    return 0;
+}
+
+int ipic_gpu_upload_fields(ipic_ctx* ctx, const double* E, const double* Bc) {
+    if (int rc = check_ctx(ctx)) return rc;
+    const Geo& g = ctx->g;
+    const int nc = g.ncell;
+    const size_t n3 = (size_t)nc * 3;
+    // total fields: the given ones, or the current state for a NULL argument
+    std::vector<double> tE(n3), tB(n3);
+    if (!E || !Bc)
+        if (int rc = total_fields(ctx, E ? nullptr : tE.data(), Bc ? nullptr : tB.data())) return rc;
+    if (E) std::memcpy(tE.data(), E, n3 * sizeof(double));
+    if (Bc) std::memcpy(tB.data(), Bc, n3 * sizeof(double));
+    for (size_t i = 0; i < n3; ++i)
+        if (!std::isfinite(tE[i]) || !std::isfinite(tB[i])) return fail(ctx, IPIC_E_ARG, "non-finite field value");
+    ctx->bgE.swap(tE);
+    ctx->bgBc.swap(tB);
+    // uniform parts: mean over the grid
+    double e0[3] = {0, 0, 0}, b0[3] = {0, 0, 0};
+    for (int i = 0; i < nc; ++i)
+        for (int k = 0; k < 3; ++k) {
+            e0[k] += ctx->bgE[3 * i + k];
+            b0[k] += ctx->bgBc[3 * i + k];
+        }
+    for (int k = 0; k < 3; ++k) {
+        e0[k] /= nc;
+        b0[k] /= nc;
+        ctx->g.E0[k] = (float)e0[k];
+        ctx->g.B0[k] = (float)b0[k];
+    }
+    // non-uniform remainders relative to the FP32-rounded uniform parts
+    std::vector<double> rE(n3), rB(n3);
+    bool nuE = false, nuB = false;
+    for (int i = 0; i < nc; ++i)
+        for (int k = 0; k < 3; ++k) {
+            rE[3 * i + k] = ctx->bgE[3 * i + k] - (double)ctx->g.E0[k];
+            rB[3 * i + k] = ctx->bgBc[3 * i + k] - (double)ctx->g.B0[k];
+            nuE |= rE[3 * i + k] != 0.0;
+            nuB |= rB[3 * i + k] != 0.0;
+        }
+    // static sources of the background: SE = c^2 curl- Bc_bg (field.h:359-363),
+    // SB = curl+ E_bg (field.h:367-371), FP64
+    std::vector<double> sE(n3), sB(n3), rBn(n3);
+    const int* n = g.n;
+    const double* w = ctx->d.cell_width;
+    const double c2 = ctx->d.c * ctx->d.c;
+    auto id = [&](int x, int y, int z) {
+        x = (x % n[0] + n[0]) % n[0];
+        y = (y % n[1] + n[1]) % n[1];
+        z = (z % n[2] + n[2]) % n[2];
+        return (size_t)((x * n[1] + y) * n[2] + z);
+    };
+    bool hasSE = false, hasSB = false;
+    const std::vector<double>& bB = ctx->bgBc;
+    const std::vector<double>& bE = ctx->bgE;
+    for (int x = 0; x < n[0]; ++x)
+        for (int y = 0; y < n[1]; ++y)
+            for (int z = 0; z < n[2]; ++z) {
+                const size_t i = id(x, y, z), mx = id(x - 1, y, z), my = id(x, y - 1, z), mz = id(x, y, z - 1);
+                const size_t px = id(x + 1, y, z), py = id(x, y + 1, z), pz = id(x, y, z + 1);
+                sE[3 * i + 0] = c2 * ((bB[3 * i + 2] - bB[3 * my + 2]) / w[1] - (bB[3 * i + 1] - bB[3 * mz + 1]) / w[2]);
+                sE[3 * i + 1] = c2 * ((bB[3 * i + 0] - bB[3 * mz + 0]) / w[2] - (bB[3 * i + 2] - bB[3 * mx + 2]) / w[0]);
+                sE[3 * i + 2] = c2 * ((bB[3 * i + 1] - bB[3 * mx + 1]) / w[0] - (bB[3 * i + 0] - bB[3 * my + 0]) / w[1]);
+                sB[3 * i + 0] = (bE[3 * py + 2] - bE[3 * i + 2]) / w[1] - (bE[3 * pz + 1] - bE[3 * i + 1]) / w[2];
+                sB[3 * i + 1] = (bE[3 * pz + 0] - bE[3 * i + 0]) / w[2] - (bE[3 * px + 2] - bE[3 * i + 2]) / w[0];
+                sB[3 * i + 2] = (bE[3 * px + 1] - bE[3 * i + 1]) / w[0] - (bE[3 * py + 0] - bE[3 * i + 0]) / w[1];
+                for (int k = 0; k < 3; ++k) {
+                    hasSE |= sE[3 * i + k] != 0.0;
+                    hasSB |= sB[3 * i + k] != 0.0;
+                }
+                for (int k = 0; k < 3; ++k) {
+                    double acc = 0;
+                    for (int a = 0; a < 2; ++a)
+                        for (int b = 0; b < 2; ++b)
+                            for (int c = 0; c < 2; ++c) acc += rB[3 * id(x - a, y - b, z - c) + k];
+                    rBn[3 * i + k] = 0.125 * acc;
+                }
+            }
+    auto put = [&](float4** dst, bool need, const std::vector<double>& src) -> int {
+        if (!need) {
+            dfree(ctx, *dst);
+            *dst = nullptr;
+            return 0;
+        }
+        if (!*dst)
+            if (int rc = dalloc(ctx, dst, (size_t)nc)) return rc;
+        std::vector<float4> h(nc);
+        for (int i = 0; i < nc; ++i)
+            h[i] = make_float4((float)src[3 * i], (float)src[3 * i + 1], (float)src[3 * i + 2], 0.0f);
+        HIPCK(ctx, hipMemcpyAsync(*dst, h.data(), nc * sizeof(float4), hipMemcpyHostToDevice, ctx->stream));
+        HIPCK(ctx, hipStreamSynchronize(ctx->stream));
+        return 0;
+    };
+    if (int rc = put(&ctx->Ebg, nuE, rE)) return rc;
+    if (int rc = put(&ctx->Eg, nuE, rE)) return rc;  // gather E = dE + Ebg with dE = 0
+    if (int rc = put(&ctx->Bcbg, nuB, rB)) return rc;
+    if (int rc = put(&ctx->Bnbg, nuB, rBn)) return rc;
+    if (int rc = put(&ctx->SE, hasSE, sE)) return rc;
+    if (int rc = put(&ctx->SB, hasSB, sB)) return rc;
+    // perturbations restart from zero; gather B = interpC2N(Bcbg)
+    HIPCK(ctx, hipMemsetAsync(ctx->E, 0, nc * sizeof(float4), ctx->stream));
+    HIPCK(ctx, hipMemsetAsync(ctx->Bc, 0, nc * sizeof(float4), ctx->stream));
+    HIPCK(ctx, hipMemsetAsync(ctx->Bn, 0, nc * sizeof(float4), ctx->stream));
+    if (nuB) HIPCK(ctx, hipMemcpyAsync(ctx->Bn, ctx->Bnbg, nc * sizeof(float4), hipMemcpyDeviceToDevice, ctx->stream));
+    HIPCK(ctx, hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int ipic_gpu_step(ipic_ctx* ctx, int64_t nsteps, ipic_step_stats* stats) {
+    if (int rc = check_ctx(ctx)) return rc;
+    if (nsteps < 0) return fail(ctx, IPIC_E_ARG, "negative step count");
+    const bool timing = (ctx->d.flags & IPIC_FLAG_PHASE_TIMING) != 0;
+    ipic_step_stats st{};
+    HIPCK(ctx, hipMemsetAsync(ctx->removed, 0, sizeof(unsigned long long), ctx->stream));
+    HIPCK(ctx, hipStreamSynchronize(ctx->stream));
+    double ms_field = 0, ms_push = 0, ms_scan = 0, ms_dep = 0;
+    auto t0 = std::chrono::steady_clock::now();
+    auto t1 = t0;
+    for (int64_t s = 0; s < nsteps; ++s) {
+        if (!ctx->j_valid) {
+            if (timing) HIPCK(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
+            if (int rc = launch_particles(ctx, MODE_DEPOSIT)) return rc;
+            if (timing) {
+                HIPCK(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
+                HIPCK(ctx, hipEventSynchronize(ctx->ev[1]));
+                float ms = 0;
+                HIPCK(ctx, hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+                ms_dep += ms;
+            }
+            ctx->j_valid = true;
+        }
+        if (timing) HIPCK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
+        if (int rc = launch_field(ctx)) return rc;
+        if (timing) HIPCK(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
+        if (int rc = launch_particles(ctx, MODE_PUSH)) return rc;
+        if (timing) HIPCK(ctx, hipEventRecord(ctx->ev[4], ctx->stream));
+        if (int rc = launch_scan(ctx)) return rc;
+        if (timing) {
+            HIPCK(ctx, hipEventRecord(ctx->ev[5], ctx->stream));
+            HIPCK(ctx, hipEventSynchronize(ctx->ev[5]));
+            float a = 0, b = 0, c = 0;
+            HIPCK(ctx, hipEventElapsedTime(&a, ctx->ev[2], ctx->ev[3]));
+            HIPCK(ctx, hipEventElapsedTime(&b, ctx->ev[3], ctx->ev[4]));
+            HIPCK(ctx, hipEventElapsedTime(&c, ctx->ev[4], ctx->ev[5]));
+            ms_field += a;
+            ms_push += b;
+            ms_scan += c;
+        }
+        ctx->canonical = false;
+        if (s == 0) {
+            HIPCK(ctx, hipStreamSynchronize(ctx->stream));
+            t1 = std::chrono::steady_clock::now();
+        }
+    }
+    HIPCK(ctx, hipStreamSynchronize(ctx->stream));
+    auto t2 = std::chrono::steady_clock::now();
+    if (int rc = check_device_flag(ctx)) return rc;
+    unsigned long long removed = 0;
+    HIPCK(ctx, hipMemcpy(&removed, ctx->removed, sizeof(removed), hipMemcpyDeviceToHost));
+    st.first_step_s = std::chrono::duration<double>(t1 - t0).count();
+    st.remaining_steps_s = std::chrono::duration<double>(t2 - t1).count();
+    st.steps = nsteps;
+    st.removed = (int64_t)removed;
+    st.ms_field = ms_field;
+    st.ms_push = ms_push;
+    st.ms_scan = ms_scan;
+    st.ms_deposit = ms_dep;
+    long long tot = 0;
+    for (int s = 0; s < ctx->d.nspecies; ++s) {
+        long long t = 0;
+        if (int rc = read_total(ctx, s, &t)) return rc;
+        tot += t;
+    }
+    st.particles = tot;
+    if (stats) *stats = st;
+    return 0;
+}
+
+int ipic_gpu_num_particles(ipic_ctx* ctx, int32_t species, int64_t* n) {
+    if (int rc = check_ctx(ctx)) return rc;
+    if (!n) return fail(ctx, IPIC_E_ARG, "null output");
+    if (species >= ctx->d.nspecies) return fail(ctx, IPIC_E_ARG, "species out of range");
+    if (int rc = canonicalize(ctx)) return rc;
+    long long tot = 0;
+    for (int s = 0; s < ctx->d.nspecies; ++s) {
+        if (species >= 0 && s != species) continue;
+        long long t = 0;
+        if (int rc = read_total(ctx, s, &t)) return rc;
+        tot += t;
+    }
+    *n = tot;
+    return 0;
+}
+
+int ipic_gpu_cell_counts(ipic_ctx* ctx, int32_t species, int64_t* counts) {
+    if (int rc = check_ctx(ctx)) return rc;
+    if (!counts) return fail(ctx, IPIC_E_ARG, "null output");
+    if (species >= ctx->d.nspecies) return fail(ctx, IPIC_E_ARG, "species out of range");
+    if (int rc = canonicalize(ctx)) return rc;
+    const Geo& g = ctx->g;
+    const int nc = g.ncell;
+    std::vector<int> st(nc + 1);
+    std::memset(counts, 0, (size_t)nc * sizeof(int64_t));
+    for (int s = 0; s < ctx->d.nspecies; ++s) {
+        if (species >= 0 && s != species) continue;
+        HIPCK(ctx, hipMemcpy(st.data(), ctx->sp[s].start, (nc + 1) * sizeof(int), hipMemcpyDeviceToHost));
+        for (int cid = 0; cid < nc; ++cid) {
+            int x, y, z;
+            cid_xyz(g, cid, x, y, z);
+            counts[(x * g.n[1] + y) * g.n[2] + z] += st[cid + 1] - st[cid];
+        }
+    }
+    return 0;
+}
+
+int ipic_gpu_download_particles(ipic_ctx* ctx, int32_t species, double* pos, double* vel, int64_t* cell,
+                                uint32_t* ids) {
+    if (int rc = check_ctx(ctx)) return rc;
+    if (species < 0 || species >= ctx->d.nspecies) return fail(ctx, IPIC_E_ARG, "species out of range");
+    if (ids && !ctx->track) return fail(ctx, IPIC_E_STATE, "ids need IPIC_FLAG_TRACK_IDS");
+    if (int rc = canonicalize(ctx)) return rc;
+    const Geo& g = ctx->g;
+    const int nc = g.ncell;
+    Species& S = ctx->sp[species];
+    std::vector<int> st(nc + 1);
+    HIPCK(ctx, hipMemcpy(st.data(), S.start, (nc + 1) * sizeof(int), hipMemcpyDeviceToHost));
+    const long long n = st[nc];
+    std::vector<float> X((size_t)6 * std::max<long long>(n, 1));
+    for (int k = 0; k < 6 && n > 0; ++k)
+        HIPCK(ctx, hipMemcpy(&X[(size_t)k * n], S.X[S.cur] + (size_t)k * S.cap, (size_t)n * sizeof(float),
+                             hipMemcpyDeviceToHost));
+    std::vector<uint32_t> hid(ids ? std::max<long long>(n, 1) : 0);
+    if (ids && n > 0) HIPCK(ctx, hipMemcpy(hid.data(), S.ids[S.cur], (size_t)n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    long long o = 0;
+    for (int x = 0; x < g.n[0]; ++x)
+        for (int y = 0; y < g.n[1]; ++y)
+            for (int z = 0; z < g.n[2]; ++z) {
+                const int cid = host_cid(g, x, y, z);
+                const int cc[3] = {x, y, z};
+                for (int p = st[cid]; p < st[cid + 1]; ++p, ++o) {
+                    for (int k = 0; k < 3; ++k) {
+                        if (pos) pos[3 * o + k] = ctx->d.origin[k] + ((double)cc[k] + (double)X[(size_t)k * n + p]) * ctx->d.cell_width[k];
+                        if (vel) vel[3 * o + k] = (double)X[(size_t)(3 + k) * n + p];
+                    }
+                    if (cell) cell[o] = ((long long)x * g.n[1] + y) * g.n[2] + z;
+                    if (ids) ids[o] = hid[p];
+                }
+            }
+    return 0;
+}
+
+int ipic_gpu_download_fields(ipic_ctx* ctx, double* E, double* Bc, double* Bn, double* J) {
+    if (int rc = check_ctx(ctx)) return rc;
+    const Geo& g = ctx->g;
+    const int nc = g.ncell;
+    std::vector<double> tB;
+    if (Bn && !Bc) tB.resize((size_t)nc * 3);
+    double* bc = Bc ? Bc : (Bn ? tB.data() : nullptr);
+    if (int rc = total_fields(ctx, E, bc)) return rc;
+    if (Bn) {
+        // interpC2N (field.h:234-247) of the total Bc, FP64
+        const int* n = g.n;
+        for (int x = 0; x < n[0]; ++x)
+            for (int y = 0; y < n[1]; ++y)
+                for (int z = 0; z < n[2]; ++z) {
+                    const size_t i = (size_t)((x * n[1] + y) * n[2] + z);
+                    for (int k = 0; k < 3; ++k) {
+                        double acc = 0;
+                        for (int a = 0; a < 2; ++a)
+                            for (int b = 0; b < 2; ++b)
+                                for (int c = 0; c < 2; ++c) {
+                                    const int xx = (x - a + n[0]) % n[0], yy = (y - b + n[1]) % n[1],
+                                              zz = (z - c + n[2]) % n[2];
+                                    acc += bc[3 * ((xx * n[1] + yy) * n[2] + zz) + k];
+                                }
+                        Bn[3 * i + k] = 0.125 * acc;
+                    }
+                }
+    }
+    if (J) {
+        std::vector<float4> h(nc);
+        HIPCK(ctx, hipMemcpy(h.data(), ctx->J, nc * sizeof(float4), hipMemcpyDeviceToHost));
+        for (int i = 0; i < nc; ++i) {
+            J[3 * i] = h[i].x;
+            J[3 * i + 1] = h[i].y;
+            J[3 * i + 2] = h[i].z;
+        }
+    }
+    return 0;
+}
+
+int ipic_gpu_conservation(ipic_ctx* ctx, ipic_conservation* out) {
+    if (int rc = check_ctx(ctx)) return rc;
+    if (!out) return fail(ctx, IPIC_E_ARG, "null output");
+    ipic_conservation r{};
+    std::vector<double> h((size_t)ctx->red_blocks * 5);
+    for (int s = 0; s < ctx->d.nspecies; ++s) {
+        Species& S = ctx->sp[s];
+        long long nslots = 0;
+        if (int rc = read_slots(ctx, s, &nslots)) return rc;
+        if (nslots == 0) continue;
+        const double mass = S.q / S.qom;
+        hipLaunchKernelGGL(k_reduce_particles, dim3(ctx->red_blocks), dim3(256), 0, ctx->stream, S.X[S.cur], S.cap,
+                           nslots, mass, ctx->red);
+        HIPCK(ctx, hipGetLastError());
+        HIPCK(ctx, hipMemcpyAsync(h.data(), ctx->red, (size_t)ctx->red_blocks * 3 * sizeof(double),
+                                  hipMemcpyDeviceToHost, ctx->stream));
+        HIPCK(ctx, hipStreamSynchronize(ctx->stream));
+        double ke = 0, mo = 0, cnt = 0;
+        for (int b = 0; b < ctx->red_blocks; ++b) {
+            ke += h[3 * b];
+            mo += h[3 * b + 1];
+            cnt += h[3 * b + 2];
+        }
+        r.kinetic_energy += ke;
+        r.momentum += mo;
+        r.charge += cnt * S.q;
+        r.particles += (int64_t)cnt;
+    }
+    FieldArgs F = field_args(ctx);
+    hipLaunchKernelGGL(k_reduce_fields, dim3(ctx->red_blocks), dim3(256), 0, ctx->stream, F, ctx->red);
+    HIPCK(ctx, hipGetLastError());
+    HIPCK(ctx, hipMemcpyAsync(h.data(), ctx->red, (size_t)ctx->red_blocks * 5 * sizeof(double), hipMemcpyDeviceToHost,
+                              ctx->stream));
+    HIPCK(ctx, hipStreamSynchronize(ctx->stream));
+    double se = 0, sb = 0, sej = 0, sd2 = 0, md = 0;
+    for (int b = 0; b < ctx->red_blocks; ++b) {
+        se += h[5 * b];
+        sb += h[5 * b + 1];
+        sej += h[5 * b + 2];
+        sd2 += h[5 * b + 3];
+        md = std::max(md, h[5 * b + 4]);
+    }
+    const double vol = ctx->d.cell_width[0] * ctx->d.cell_width[1] * ctx->d.cell_width[2];
+    r.field_energy_E = 0.5 * se * vol;
+    r.field_energy_B = 0.5 * ctx->d.c * ctx->d.c * sb * vol;
+    r.EdotJ = sej * vol;
+    r.divB_max = md;
+    r.divB_l2 = std::sqrt(sd2);
+    *out = r;
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,153 @@
+/*
+ * ipic_gpu.h -- C ABI of the MI355X (gfx950) particle-in-cell step library libipic_gpu.so.
+ *
+ * This is the drop-in boundary for the reference's step API:
+ *   template<P,F,M> DurationMeasurement simulateSteps(std::uint64_t, Universe&)
+ *   (code/app/include/ipic3d/app/simulator.h:42-47, 127-242)
+ * with the projector / field solver / mover policies of simulator.h:244-276 replaced by
+ * the GPU implementation.  The C++ mirror in include/ipic3d_gpu/ipic3d.hpp binds it to a
+ * Universe (universe.h:12-51); any other host language binds these plain-C entry points.
+ *
+ * Conventions: plain pointers and sizes, no C++ types; every call returns 0 on success and
+ * a negative IPIC_E_* code on failure, with a message from ipic_gpu_last_error(ctx)
+ * (ctx may be NULL for a failed create).  No exception crosses the ABI.
+ *
+ * Grid conventions: the physics grid is N-periodic (nx*ny*nz cells).  Node, centre and
+ * cell arrays are row-major with index (x*ny + y)*nz + z and 3 doubles per entry.
+ *   E  -- electric field on nodes       (reference Field::E at index pos+1, field.h:14-18)
+ *   Bc -- magnetic field on cell centres (reference BcField::Bc at pos+1, field.h:20-23)
+ *   Bn -- magnetic field on nodes = 1/8 sum of the 8 surrounding Bc (field.h:234-247)
+ *   J  -- current density on nodes      (reference CurrentDensity::J at pos, field.h:25-27)
+ */
+#ifndef IPIC_GPU_H
+#define IPIC_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define IPIC_MAX_SPECIES 8
+
+#define IPIC_OK 0
+#define IPIC_E_ARG -1      /* invalid argument */
+#define IPIC_E_HIP -2      /* HIP runtime error (no device, allocation, launch) */
+#define IPIC_E_STATE -3    /* call not valid in the current state */
+#define IPIC_E_RANGE -4    /* a size exceeds a documented limit */
+
+/* flags */
+#define IPIC_FLAG_TRACK_IDS 1u    /* carry a uint32 id per particle (input order) */
+#define IPIC_FLAG_NO_GRAPH 2u     /* launch kernels one by one instead of a HIP graph */
+#define IPIC_FLAG_PHASE_TIMING 4u /* record per-phase HIP events (implies NO_GRAPH) */
+
+/* Particle species: every particle of a species shares Particle::q and Particle::qom
+ * (particle.h:15-16). */
+typedef struct ipic_species_desc {
+    double q;
+    double qom;
+} ipic_species_desc;
+
+/* UniverseProperties (universe_properties.h:52-142) plus the GPU layout parameters. */
+typedef struct ipic_desc {
+    int32_t n[3];                 /* cells per axis (UniverseProperties::size) */
+    int32_t nspecies;             /* 1..IPIC_MAX_SPECIES */
+    double cell_width[3];         /* UniverseProperties::cellWidth */
+    double origin[3];             /* UniverseProperties::origin */
+    double dt;                    /* UniverseProperties::dt */
+    double c;                     /* UniverseProperties::speedOfLight */
+    ipic_species_desc species[IPIC_MAX_SPECIES];
+    int32_t field_substeps;       /* leapfrog substeps per particle step with J held fixed (>= 1) */
+    int32_t sink;                 /* 1: remove particles with |x - sink_center|^2 <= sink_radius^2
+                                     (cell.h:1123-1128, planetRadius / objectCenter) */
+    double sink_center[3];
+    double sink_radius;
+    int32_t tile[3];              /* cells per tile along each axis; 0 = automatic (<= 8, divides n) */
+    uint32_t flags;               /* IPIC_FLAG_* */
+} ipic_desc;
+
+/* DurationMeasurement (simulator.h:37-40) plus counters of the call. */
+typedef struct ipic_step_stats {
+    double first_step_s;          /* wall time of the first step of the call */
+    double remaining_steps_s;     /* wall time of the other steps */
+    int64_t steps;
+    int64_t particles;            /* live particles after the call, all species */
+    int64_t removed;              /* particles removed by the sink during the call */
+    /* filled when IPIC_FLAG_PHASE_TIMING is set: summed GPU milliseconds per phase */
+    double ms_field;              /* leapfrog E/B passes, interpC2N, cross-tile J sum */
+    double ms_push;               /* fused gather + Boris + rebin + deposit kernel */
+    double ms_scan;               /* cell-count prefix sums */
+    double ms_deposit;            /* initial deposit of a fresh particle state */
+} ipic_step_stats;
+
+/* FP64 conservation sums, reduced on the GPU (cell.h:1315-1355, field.h:443-459). */
+typedef struct ipic_conservation {
+    double kinetic_energy;        /* sum 0.5 (q/qom) |v|^2      (cell.h:1315-1324) */
+    double momentum;              /* sum (q/qom) |v|            (cell.h:1329-1338) */
+    double charge;                /* sum q */
+    double field_energy_E;        /* 0.5 * sum |E|^2 * vol over nodes */
+    double field_energy_B;        /* 0.5 * c^2 * sum |Bc|^2 * vol over centres */
+    double divB_max;              /* max |div+ Bc| on the index lattice */
+    double divB_l2;               /* l2 norm of div+ Bc */
+    double EdotJ;                 /* sum E . J * vol over nodes (work of the field on J) */
+    int64_t particles;            /* live particles (countParticlesInDomain, cell.h:87-100) */
+} ipic_conservation;
+
+typedef struct ipic_ctx ipic_ctx;
+
+/* Universe(properties) (universe.h:40-51): allocates device state for the grid and species.
+ * Fields start at zero; no particles. */
+int ipic_gpu_create(const ipic_desc* desc, ipic_ctx** out);
+int ipic_gpu_destroy(ipic_ctx* ctx);
+const char* ipic_gpu_last_error(const ipic_ctx* ctx);
+
+/* Replaces the particles of one species (Cell::particles of every cell, cell.h:31-38).
+ * pos, vel: n x 3 doubles, absolute positions (Particle::position / velocity).  Positions
+ * outside the box are wrapped periodically; each particle is binned into the cell
+ * floor((x - origin) / width). */
+int ipic_gpu_upload_particles(ipic_ctx* ctx, int32_t species, int64_t n,
+                              const double* pos, const double* vel);
+
+/* Synthetic initialisation on the GPU (the role of initCells, cell.h:700-776): for every
+ * cell, ppc[cell] particles (ppc == NULL: uniform_ppc each) with offsets uniform in the
+ * cell and velocities drift + vth * N(0,1) per component, drawn from the counter-based
+ * generator keyed by (seed, species, cell, index). */
+int ipic_gpu_init_particles(ipic_ctx* ctx, int32_t species, const int32_t* ppc, int32_t uniform_ppc,
+                            const double vth[3], const double drift[3], uint64_t seed);
+
+/* Field (nodes) and BcField (centres) state, n^3 x 3 doubles each; NULL leaves one as is. */
+int ipic_gpu_upload_fields(ipic_ctx* ctx, const double* E, const double* Bc);
+
+/* simulateSteps (simulator.h:127-242) with the intended full loop of simulator.h:189-223:
+ * per step deposit J (cell.h:787-824), leapfrog E then Bc (field.h:356-372) and
+ * interpC2N (field.h:234-247), then gather + Boris move (cell.h:896-964) and rebin
+ * (cell.h:975-1310) with periodic boundaries.  stats may be NULL. */
+int ipic_gpu_step(ipic_ctx* ctx, int64_t nsteps, ipic_step_stats* stats);
+
+/* countParticlesInDomain (cell.h:87-100) per species (species < 0: all). */
+int ipic_gpu_num_particles(ipic_ctx* ctx, int32_t species, int64_t* n);
+
+/* Per-cell particle counts (outputNumberOfParticlesPerCell, cell.h:1360-1395), ncell int64,
+ * species < 0: all species. */
+int ipic_gpu_cell_counts(ipic_ctx* ctx, int32_t species, int64_t* counts);
+
+/* Particles of one species sorted by cell (row-major cell index), each cell's particles in
+ * the order held on the device.  pos, vel: n x 3 doubles (absolute); cell, ids nullable
+ * (ids need IPIC_FLAG_TRACK_IDS). */
+int ipic_gpu_download_particles(ipic_ctx* ctx, int32_t species, double* pos, double* vel,
+                                int64_t* cell, uint32_t* ids);
+
+/* Field state; any pointer may be NULL.  J is the current density used by the last
+ * field update. */
+int ipic_gpu_download_fields(ipic_ctx* ctx, double* E, double* Bc, double* Bn, double* J);
+
+/* FP64 conservation sums of the current state. */
+int ipic_gpu_conservation(ipic_ctx* ctx, ipic_conservation* out);
+
+/* Device properties used by the tests and benchmark. */
+int ipic_gpu_device_info(char* name, int32_t name_len, int32_t* cu_count, int64_t* hbm_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,46 @@
+"""CPU-side checks of the drop-in boundary: the C-ABI library loads (no GPU needed for
+dlopen) and exports every entry point include/ipic_gpu.h declares; the host package raises
+instead of falling back when the library is absent."""
+import ctypes
+import os
+import re
+
+import pytest
+
+import allscale_ipic3d_amd as G
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared(header):
+    txt = open(os.path.join(ROOT, "include", header)).read()
+    return sorted(set(re.findall(r"^(?:int|const char\*)\s+(ipic_\w+)\s*\(", txt, flags=re.M)))
+
+
+def test_header_declares_binding_symbols():
+    assert _declared("ipic_gpu.h") == sorted(G.ABI_SYMBOLS)
+
+
+@pytest.mark.skipif(not os.path.exists(G.LIB_PATH), reason="library not built")
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(G.LIB_PATH)
+    for name in _declared("ipic_gpu.h"):
+        assert hasattr(lib, name), name
+
+
+def test_missing_library_raises(tmp_path):
+    saved = G._lib
+    G._lib = None
+    try:
+        with pytest.raises(G.IpicError):
+            G.load_library(str(tmp_path / "nope.so"))
+    finally:
+        G._lib = saved
+
+
+def test_desc_layout_matches_header():
+    # ipic_desc: 4 int32, 3+3 doubles, 2 doubles, 8 species x 2 doubles, 2 int32, 3 doubles,
+    # 1 double, 3 int32, 1 uint32
+    assert ctypes.sizeof(G.Desc) == 16 + 48 + 16 + 128 + 8 + 24 + 8 + 16
+    assert ctypes.sizeof(G.StepStats) == 9 * 8
+    assert ctypes.sizeof(G.Conservation) == 9 * 8
