@@ -30,8 +30,11 @@
 namespace {
 
 constexpr int NT = 256;          // threads per block of the particle kernels
-constexpr int KRUN = 8;          // max particles per run (one lane)
-constexpr int CH = NT * KRUN;    // max particles staged in LDS per chunk
+constexpr int WS = 64;           // wavefront size
+constexpr int NW = NT / WS;      // waves per block
+constexpr int KRUN = 7;          // max particles per run (one lane, held in registers)
+constexpr int WCH = WS * KRUN;   // max particles per wave chunk
+constexpr int K0CAP = 64;        // leaver buffer entries per wave chunk
 constexpr int TCMAX = 512;       // max cells per tile
 constexpr int NJMAX = 11 * 11 * 11;  // max J nodes per tile ((T+3)^3, T <= 8)
 constexpr int JPMAX = 4;         // max (tile, local) pairs per axis coordinate
@@ -48,6 +51,7 @@ struct Geo {
     int NJ;
     int nj[3];
     int nmax;
+    int s0;               // n[1] * n[2], row stride of x
     float w[3];
     float inv_w[3];
     float origin[3];
@@ -59,11 +63,15 @@ struct Geo {
     float sink_r2;
     float E0[3];          // uniform part of the background field (added in the gather)
     float B0[3];
+    float inv_T[3];       // 1 / T (exact small-integer division by a float reciprocal)
+    float inv_T12;        // 1 / (T[1] * T[2])
+    float fix_pre;        // 2^(fix_bits - 32): J in fixed point with fix_bits fractional bits
+    double fix_inv;       // 2^-fix_bits
 };
 
 struct SpArgs {
-    const float* in;      // 6 arrays of cap floats: ox oy oz vx vy vz
-    float* out;
+    const float* in[6];   // input arrays: offsets ox oy oz (cell units), velocities vx vy vz
+    float* out[6];        // output arrays, same order
     const uint32_t* id_in;
     uint32_t* id_out;
     long long cap;
@@ -75,6 +83,7 @@ struct SpArgs {
     int* stay_acc;        // next-step stayers by output segment
     float qom;
     float qs;             // q / (8 vol)
+    float subk;           // 4 |qom| dt / (pi c): sub-cycle ratio per unit |B|^2
 };
 
 struct PushArgs {
@@ -107,8 +116,12 @@ __device__ __forceinline__ int wrapc(int i, int n) {
     return r < 0 ? r + n : r;
 }
 
+// floor(a / b) for 0 <= a < 2^16 and b >= 1 through the rounded reciprocal inv_b of b: the
+// quotient is offset by 0.5/b, more than the rounding error a * 2^-22 of the product.
+__device__ __forceinline__ int fdivi(int a, float inv_b) { return (int)(((float)a + 0.5f) * inv_b); }
+
 __device__ __forceinline__ int cid_of(const Geo& g, int x, int y, int z) {
-    const int tx = x / g.T[0], ty = y / g.T[1], tz = z / g.T[2];
+    const int tx = fdivi(x, g.inv_T[0]), ty = fdivi(y, g.inv_T[1]), tz = fdivi(z, g.inv_T[2]);
     const int lx = x - tx * g.T[0], ly = y - ty * g.T[1], lz = z - tz * g.T[2];
     return ((tx * g.NTl[1] + ty) * g.NTl[2] + tz) * g.TC + (lx * g.T[1] + ly) * g.T[2] + lz;
 }
@@ -149,8 +162,19 @@ __device__ __forceinline__ int block_exscan(int v, int* s_w, int& total) {
     return off + x - v;
 }
 
-__device__ __forceinline__ void lds_add(float* p, float v) {
-    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+// J is accumulated in LDS as 64-bit fixed point (fix_bits fractional bits): integer atomics
+// are fast where float atomics on shared addresses serialise, and the sum is exact and
+// independent of the order of the adds.  |v| * 2^fix_bits = hi * 2^32 + lo with
+// a = |v| * 2^(fix_bits - 32), hi = floor(a) and lo = (a - hi) * 2^32 rounded to nearest:
+// a - hi is exact for a >= 0 and at most 1 - 2^-24, so lo < 2^32; the sign is applied to the
+// integer, which keeps the rounding symmetric (unbiased).
+__device__ __forceinline__ void fix_add(long long* p, float v, float pre) {
+    const float a = fminf(fabsf(v) * pre, 1073741823.0f);
+    const float h = floorf(a);
+    const unsigned lo = (unsigned)rintf((a - h) * 4294967296.0f);
+    long long q = (long long)(((unsigned long long)(unsigned)h << 32) + lo);
+    if (v < 0.0f) q = -q;
+    __hip_atomic_fetch_add(p, q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
 // Trilinear weights (cell.h:876-880): w[(i*2+j)*2+k] = (i?x:1-x)(j?y:1-y)(k?z:1-z)
@@ -165,7 +189,7 @@ __device__ __forceinline__ void tri_weights(float x, float y, float z, float wt[
 
 // Deposit of one particle outside the run's own cell: corner (i,j,k) of cell e goes to the
 // tile's LDS accumulator when inside its halo, else to the global overflow accumulator.
-__device__ __forceinline__ void deposit_direct(const PushArgs& A, float* sJ, int x0, int y0, int z0,
+__device__ __forceinline__ void deposit_direct(const PushArgs& A, long long* sJ, int x0, int y0, int z0,
                                                int elx, int ely, int elz, const float wt[8], float jx,
                                                float jy, float jz) {
     const Geo& g = A.g;
@@ -179,9 +203,9 @@ __device__ __forceinline__ void deposit_direct(const PushArgs& A, float* sJ, int
                 const int nx = elx + 1 + i, ny = ely + 1 + j, nz = elz + 1 + k;
                 if (nx >= 0 && nx < g.nj[0] && ny >= 0 && ny < g.nj[1] && nz >= 0 && nz < g.nj[2]) {
                     const int l = (nx * g.nj[1] + ny) * g.nj[2] + nz;
-                    lds_add(&sJ[l], jx * ww);
-                    lds_add(&sJ[g.NJ + l], jy * ww);
-                    lds_add(&sJ[2 * g.NJ + l], jz * ww);
+                    fix_add(&sJ[l], jx * ww, g.fix_pre);
+                    fix_add(&sJ[g.NJ + l], jy * ww, g.fix_pre);
+                    fix_add(&sJ[2 * g.NJ + l], jz * ww, g.fix_pre);
                 } else {
                     const int gid = nid_of(g, wrapc(x0 + elx + i, g.n[0]), wrapc(y0 + ely + j, g.n[1]),
                                            wrapc(z0 + elz + k, g.n[2]));
@@ -205,24 +229,208 @@ __device__ __forceinline__ void count_add(const Geo& g, int* s_arr, int* gl, int
 
 // ---------------------------------------------------------------------------------------------
 // particle kernel: PUSH (gather + Boris + rebin + deposit), DEPOSIT (J only), REBIN (sort only)
+//
+// One block per tile of cells.  The tile's cells are split among the block's waves by particle
+// count and every wave streams its cells on its own: it cuts them into runs of at most KRUN
+// particles of one cell, one run per lane, and each lane loads its run straight into registers
+// (all its loads in flight at once).  The fast path covers particles that start and end the
+// step in the run's cell: the gather evaluates per-run trilinear coefficients and the deposit
+// accumulates per-run moments, both in registers; the updated particle goes directly to its
+// slot of the output array.  Every other particle (it sits in another cell, or leaves the
+// cell) is appended to a wave queue by ballot compaction and handled afterwards with all lanes
+// busy.  The tile's J accumulator (64-bit fixed point) and per-cell counters live in LDS.
 // ---------------------------------------------------------------------------------------------
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ int wave_incl_scan(int x, int lane) {
+#pragma unroll
+    for (int d = 1; d < WS; d <<= 1) {
+        const int y = __shfl_up(x, d, WS);
+        if (lane >= d) x += y;
+    }
+    return x;
+}
+
+__device__ __forceinline__ int wave_max(int x) {
+#pragma unroll
+    for (int d = WS / 2; d > 0; d >>= 1) x = max(x, __shfl_xor(x, d, WS));
+    return x;
+}
+
+__device__ __forceinline__ int wrap1(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
+
+// xyz of a float4 node value (a 12-byte load: the fourth component is padding)
+__device__ __forceinline__ float3 ld3(const float4* p) {
+    struct F3 { float x, y, z; };
+    const F3 v = *reinterpret_cast<const F3*>(p);
+    return make_float3(v.x, v.y, v.z);
+}
+
+__device__ __forceinline__ int ballot_rank(unsigned long long m) {
+    return __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+}
+
+// Trilinear interpolation written as the polynomial it expands to:
+// f(x,y,z) = c0 + c1 x + c2 y + c3 z + c4 xy + c5 xz + c6 yz + c7 xyz for corner values
+// f[(i*2+j)*2+k] with the weights (i?x:1-x)(j?y:1-y)(k?z:1-z) of cell.h:876-880.
+// `add` is added to every corner value and the result is multiplied by `scale`.
+__device__ __forceinline__ void tri_coef(const float f[8], float add, float scale, float c[8]) {
+    c[0] = (f[0] + add) * scale;
+    c[1] = (f[4] - f[0]) * scale;
+    c[2] = (f[2] - f[0]) * scale;
+    c[3] = (f[1] - f[0]) * scale;
+    c[4] = ((f[6] - f[4]) - (f[2] - f[0])) * scale;
+    c[5] = ((f[5] - f[4]) - (f[1] - f[0])) * scale;
+    c[6] = ((f[3] - f[2]) - (f[1] - f[0])) * scale;
+    c[7] = (((f[7] - f[6]) - (f[5] - f[4])) - ((f[3] - f[2]) - (f[1] - f[0]))) * scale;
+}
+
+__device__ __forceinline__ float tri_eval(const float c[8], float x, float y, float z) {
+    const float a = fmaf(c[1], x, c[0]);
+    const float b = fmaf(c[4], x, c[2]);
+    const float d = fmaf(c[5], x, c[3]);
+    const float e = fmaf(c[7], x, c[6]);
+    return fmaf(z, fmaf(e, y, d), fmaf(b, y, a));
+}
+
+// Moments M = sum j * {1, x, y, z, xy, xz, yz, xyz} to the 8 corner sums
+// N[(i*2+j)*2+k] = sum j * (i?x:1-x)(j?y:1-y)(k?z:1-z), one axis at a time.
+__device__ __forceinline__ void mom_to_nodes(const float M[8], float N[8]) {
+    const float a0_1 = M[0] - M[1], a1_1 = M[1];
+    const float a0_y = M[2] - M[4], a1_y = M[4];
+    const float a0_z = M[3] - M[5], a1_z = M[5];
+    const float a0_yz = M[6] - M[7], a1_yz = M[7];
+    const float b00_1 = a0_1 - a0_y, b01_1 = a0_y, b00_z = a0_z - a0_yz, b01_z = a0_yz;
+    const float b10_1 = a1_1 - a1_y, b11_1 = a1_y, b10_z = a1_z - a1_yz, b11_z = a1_yz;
+    N[0] = b00_1 - b00_z; N[1] = b00_z;
+    N[2] = b01_1 - b01_z; N[3] = b01_z;
+    N[4] = b10_1 - b10_z; N[5] = b10_z;
+    N[6] = b11_1 - b11_z; N[7] = b11_z;
+}
+
+// Adaptive sub-cycling (cell.h:948-952: n = min(100, int(dt/dt_sub)+1) with
+// dt_sub = pi c / (4 |qom| |B|^2)), Boris rotation (particle.h:31-49) and position update
+// (particle.h:24-26) in cell units.
+__device__ __forceinline__ void boris_move(const Geo& g, float qom, float subk, float Ex, float Ey, float Ez,
+                                           float Bx, float By, float Bz, float& vx, float& vy, float& vz,
+                                           float& px, float& py, float& pz) {
+    const float ratio = (Bx * Bx + By * By + Bz * Bz) * subk;
+    const int nsub = (ratio < 99.0f) ? (int)ratio + 1 : 100;
+    const float dts = (nsub == 1) ? g.dt : g.dt / (float)nsub;
+    const float kq = qom * 0.5f * dts;
+    const float sx = dts * g.inv_w[0], sy = dts * g.inv_w[1], sz = dts * g.inv_w[2];
+    const float tx = kq * Bx, ty = kq * By, tz = kq * Bz;
+    const float f = 2.0f * __builtin_amdgcn_rcpf(1.0f + (tx * tx + ty * ty + tz * tz));
+    const float ux = f * tx, uy = f * ty, uz = f * tz;
+    const float kex = kq * Ex, key = kq * Ey, kez = kq * Ez;
+    for (int sc = 0; sc < nsub; ++sc) {
+        const float mx = vx + kex, my = vy + key, mz = vz + kez;
+        const float qx = mx + (my * tz - mz * ty);
+        const float qy = my + (mz * tx - mx * tz);
+        const float qz = mz + (mx * ty - my * tx);
+        vx = mx + (qy * uz - qz * uy) + kex;
+        vy = my + (qz * ux - qx * uz) + key;
+        vz = mz + (qx * uy - qy * ux) + kez;
+        px += vx * sx;
+        py += vy * sy;
+        pz += vz * sz;
+    }
+}
+
+// E and B (gather arrays plus the uniform background, divided by the cell volume) at offsets
+// (x,y,z) of the cell with wrapped coordinates (cx,cy,cz), read straight from the node arrays.
+__device__ __forceinline__ void gather_direct(const PushArgs& A, int cx, int cy, int cz, float x, float y,
+                                              float z, float& Ex, float& Ey, float& Ez, float& Bx, float& By,
+                                              float& Bz) {
+    const Geo& g = A.g;
+    const int xs[2] = {cx * g.s0, wrap1(cx + 1, g.n[0]) * g.s0};
+    const int ys[2] = {cy * g.n[2], wrap1(cy + 1, g.n[1]) * g.n[2]};
+    const int zs[2] = {cz, wrap1(cz + 1, g.n[2])};
+    float wt[8];
+    tri_weights(x, y, z, wt);
+    Ex = Ey = Ez = Bx = By = Bz = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                const int nd = xs[i] + ys[j] + zs[k];
+                const float4 e4 = A.E[nd], b4 = A.Bn[nd];
+                const float ww = wt[(i * 2 + j) * 2 + k];
+                Ex += e4.x * ww; Ey += e4.y * ww; Ez += e4.z * ww;
+                Bx += b4.x * ww; By += b4.y * ww; Bz += b4.z * ww;
+            }
+    Ex = (Ex + g.E0[0]) * g.inv_vol; Ey = (Ey + g.E0[1]) * g.inv_vol; Ez = (Ez + g.E0[2]) * g.inv_vol;
+    Bx = (Bx + g.B0[0]) * g.inv_vol; By = (By + g.B0[1]) * g.inv_vol; Bz = (Bz + g.B0[2]) * g.inv_vol;
+}
+
+// A particle whose position is not finite after the push is removed like a sunk one: a slot
+// with a NaN offset marks a removed particle in the next pass.
+__device__ __forceinline__ bool finite3(float x, float y, float z) {
+    return __builtin_isfinite(x) && __builtin_isfinite(y) && __builtin_isfinite(z);
+}
+
+__device__ __forceinline__ bool alive(float x, float y, float z) { return x == x && y == y && z == z; }
+
+__device__ __forceinline__ bool in_cell(float x, float y, float z) {
+    return x >= 0.0f && x <= 1.0f && y >= 0.0f && y <= 1.0f && z >= 0.0f && z <= 1.0f;
+}
+
+__device__ __forceinline__ void store_particle(const SpArgs& S, int oidx, float px, float py, float pz, float vx,
+                                               float vy, float vz) {
+    S.out[0][oidx] = px;
+    S.out[1][oidx] = py;
+    S.out[2][oidx] = pz;
+    S.out[3][oidx] = vx;
+    S.out[4][oidx] = vy;
+    S.out[5][oidx] = vz;
+}
+
+// Dynamic LDS layout of k_particles (bytes, 8-aligned start)
+struct PLayout {
+    int off_seg, off_cnt, off_stay, off_wlo, off_wave, wave_bytes, total;
+};
+
+__host__ __device__ inline PLayout particle_layout(int NJ, int TC) {
+    PLayout L;
+    int o = 3 * NJ * 8;             // J, int64
+    L.off_seg = o; o += (TC + 1) * 4;
+    L.off_cnt = o; o += TC * 4;
+    L.off_stay = o; o += TC * 4;
+    L.off_wlo = o; o += (NW + 1) * 4;
+    o = (o + 15) & ~15;
+    L.off_wave = o;
+    // per wave: run table (WS+1 ints), arrival queue (WCH ushort), leaver buffer (K0CAP x 8 floats)
+    L.wave_bytes = ((WS + 1) * 4 + WCH * 2 + 15) & ~15;
+    L.wave_bytes += K0CAP * 8 * 4;
+    o += NW * L.wave_bytes;
+    L.total = o;
+    return L;
+}
 
 template <int MODE, bool TRACK>
 __global__ __launch_bounds__(NT) void k_particles(PushArgs A) {
     const Geo& g = A.g;
-    __shared__ float sP[6][CH];
-    __shared__ uint32_t sId[TRACK ? CH : 1];
-    __shared__ float sJ[3 * NJMAX];
-    __shared__ int s_seg[TCMAX + 1];
-    __shared__ int s_rb[TCMAX + 1];
-    __shared__ int s_scan[NT];
-    __shared__ int s_w[NT / 64];
-    __shared__ int s_carry[TCMAX];
-    __shared__ int s_cnt[TCMAX];
-    __shared__ int s_stay[TCMAX];
-    __shared__ int s_prange[2];
+    extern __shared__ __align__(16) unsigned char smem[];
+    const PLayout L = particle_layout(g.NJ, g.TC);
+    long long* sJ = (long long*)smem;
+    int* s_seg = (int*)(smem + L.off_seg);
+    int* s_cnt = (int*)(smem + L.off_cnt);
+    int* s_stay = (int*)(smem + L.off_stay);
+    int* s_wlo = (int*)(smem + L.off_wlo);
 
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & (WS - 1), wid = tid / WS;
+    unsigned char* wbase = smem + L.off_wave + wid * L.wave_bytes;
+    int* R = (int*)wbase;                                            // WS+1 ints
+    unsigned short* Q = (unsigned short*)(wbase + (WS + 1) * 4);     // arrival queue
+    float* K0 = (float*)(wbase + (((WS + 1) * 4 + WCH * 2 + 15) & ~15));  // leaver buffer
+
     const int t = xcd_tile(blockIdx.x, g.ntiles);
     const int tx = t / (g.NTl[1] * g.NTl[2]), ty = (t / g.NTl[2]) % g.NTl[1], tz = t % g.NTl[2];
     const int x0 = tx * g.T[0], y0 = ty * g.T[1], z0 = tz * g.T[2];
@@ -230,288 +438,352 @@ __global__ __launch_bounds__(NT) void k_particles(PushArgs A) {
     const float NaNf = __builtin_nanf("");
 
     if (MODE != MODE_REBIN)
-        for (int i = tid; i < 3 * g.NJ; i += NT) sJ[i] = 0.0f;
+        for (int i = tid; i < 3 * g.NJ; i += NT) sJ[i] = 0;
 
     for (int s = 0; s < A.nsp; ++s) {
         const SpArgs& S = A.sp[s];
+        __syncthreads();
         for (int i = tid; i <= g.TC; i += NT) s_seg[i] = S.seg[cbase + i];
         for (int i = tid; i < g.TC; i += NT) {
-            s_carry[i] = 0;
             s_cnt[i] = 0;
             s_stay[i] = 0;
         }
         __syncthreads();
-        // runs: each cell's segment is split into ceil(n/KRUN) runs of balanced length
-        int r0 = 0, r1 = 0;
-        const int c0 = 2 * tid, c1 = 2 * tid + 1;
-        if (c0 < g.TC) r0 = (s_seg[c0 + 1] - s_seg[c0] + KRUN - 1) / KRUN;
-        if (c1 < g.TC) r1 = (s_seg[c1 + 1] - s_seg[c1] + KRUN - 1) / KRUN;
-        int R;
-        const int ex0 = block_exscan(r0 + r1, s_w, R);
-        if (c0 < g.TC) s_rb[c0] = ex0;
-        if (c1 < g.TC) s_rb[c1] = ex0 + r0;
-        if (tid == 0) s_rb[g.TC] = R;
+        // cells of wave w: [s_wlo[w], s_wlo[w+1]), split at equal particle counts
+        if (tid <= NW) {
+            const long long tot = (long long)s_seg[g.TC] - s_seg[0];
+            const int target = s_seg[0] + (int)(tot * tid / NW);
+            int lo = 0, hi = g.TC;  // first cell c with s_seg[c] >= target
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (s_seg[mid] >= target) hi = mid; else lo = mid + 1;
+            }
+            s_wlo[tid] = (tid == 0) ? 0 : (tid == NW ? g.TC : lo);
+        }
         __syncthreads();
+        const int chi = s_wlo[wid + 1];
+        int c_cur = s_wlo[wid];
+        int pos_cur = s_seg[c_cur];
+        int carry = 0;  // stayers already placed for cell c_cur (when it continues)
+        const float qom = S.qom, qs = S.qs, subk = S.subk;
 
-        for (int gq = 0; gq < R; gq += NT) {
-            const int q = gq + tid;
-            const bool active = q < R;
-            int c = 0, len = 0, pst = 0;
+        while (c_cur < chi) {
+            // runs of the next 64 cells (lane l looks at cell c_cur + l)
+            const int cl = c_cur + lane;
+            int remp = 0;
+            if (cl < chi) remp = s_seg[cl + 1] - (lane == 0 ? pos_cur : s_seg[cl]);
+            const int rl = (remp + KRUN - 1) / KRUN;
+            const int incl = wave_incl_scan(rl, lane);
+            R[lane + 1] = incl;
+            if (lane == 0) R[0] = 0;
+            const int all_runs = __shfl(incl, WS - 1, WS);
+            wave_sync();
+            const int nruns = min(all_runs, WS);
+            if (nruns == 0) {  // the next 64 cells are empty
+                c_cur = min(c_cur + WS, chi);
+                pos_cur = c_cur < chi ? s_seg[c_cur] : 0;
+                carry = 0;
+                continue;
+            }
+            const bool active = lane < nruns;
+            int k = 0, len = 0, pst = 0;
             if (active) {
-                int lo = 0, hi = g.TC - 1;
+                int lo = 0, hi = WS - 1;  // largest k with R[k] <= lane
                 while (lo < hi) {
                     const int mid = (lo + hi + 1) >> 1;
-                    if (s_rb[mid] <= q) lo = mid; else hi = mid - 1;
+                    if (R[mid] <= lane) lo = mid; else hi = mid - 1;
                 }
-                c = lo;
-                const int j = q - s_rb[c];
-                const int nr = s_rb[c + 1] - s_rb[c];
-                const int nc = s_seg[c + 1] - s_seg[c];
-                const int bl = nc / nr, rem = nc - bl * nr;
+                k = lo;
+                const int j = lane - R[k];
+                const int nr = R[k + 1] - R[k];
+                const int p0 = (k == 0) ? pos_cur : s_seg[c_cur + k];
+                const int nc = s_seg[c_cur + k + 1] - p0;
+                const int bl = fdivi(nc, __builtin_amdgcn_rcpf((float)nr));
+                const int rem = nc - bl * nr;
                 len = bl + (j < rem ? 1 : 0);
-                pst = s_seg[c] + j * bl + min(j, rem);
+                pst = p0 + j * bl + min(j, rem);
             }
-            if (tid == 0) s_prange[0] = pst;
-            if (active && (q + 1 == R || tid == NT - 1)) s_prange[1] = pst + len;
-            __syncthreads();
-            const int plo = s_prange[0], nch = s_prange[1] - plo;
-            if (plo < 0 || (long long)plo + nch > S.cap || nch > CH) {
-                if (tid == 0) atomicOr(A.err, 2);
-                break;  // uniform over the block
+            const int plo = __shfl(pst, 0, WS);
+            const int phi = __shfl(pst + len, nruns - 1, WS);
+            if (plo < 0 || (long long)phi > S.cap || phi - plo > WCH || phi < plo) {  // uniform over the wave
+                if (lane == 0) atomicOr(A.err, 2);
+                break;
             }
-            for (int i = tid; i < nch; i += NT) {
+            // this lane's run into registers
+            float rx[KRUN], ry[KRUN], rz[KRUN], rvx[KRUN], rvy[KRUN], rvz[KRUN];
+            uint32_t rid[TRACK ? KRUN : 1];
 #pragma unroll
-                for (int k = 0; k < 6; ++k) sP[k][i] = S.in[(long long)k * S.cap + plo + i];
-                if (TRACK) sId[i] = S.id_in[plo + i];
+            for (int jj = 0; jj < KRUN; ++jj) {
+                if (jj < len) {
+                    const int q = pst + jj;
+                    rx[jj] = S.in[0][q];
+                    ry[jj] = S.in[1][q];
+                    rz[jj] = S.in[2][q];
+                    rvx[jj] = S.in[3][q];
+                    rvy[jj] = S.in[4][q];
+                    rvz[jj] = S.in[5][q];
+                    if (TRACK) rid[jj] = S.id_in[q];
+                } else {
+                    rx[jj] = ry[jj] = rz[jj] = NaNf;
+                    rvx[jj] = rvy[jj] = rvz[jj] = 0.0f;
+                    if (TRACK) rid[jj] = 0;
+                }
             }
-            __syncthreads();
+            const int c = c_cur + k;
+            const int lx = fdivi(c, g.inv_T12);
+            const int crem = c - lx * g.T[1] * g.T[2];
+            const int ly = fdivi(crem, g.inv_T[2]);
+            const int lz = crem - ly * g.T[2];
+            const int ax = x0 + lx, ay = y0 + ly, az = z0 + lz;
+            float cE[3][8], cB[3][8];
+            if (MODE == MODE_PUSH && active) {
+                float fe[3][8], fb[3][8];
+                const int xs[2] = {ax * g.s0, wrap1(ax + 1, g.n[0]) * g.s0};
+                const int ys[2] = {ay * g.n[2], wrap1(ay + 1, g.n[1]) * g.n[2]};
+                const int zs[2] = {az, wrap1(az + 1, g.n[2])};
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+#pragma unroll
+                    for (int j = 0; j < 2; ++j)
+#pragma unroll
+                        for (int kk = 0; kk < 2; ++kk) {
+                            const int nd = xs[i] + ys[j] + zs[kk];
+                            const float3 e3 = ld3(A.E + nd), b3 = ld3(A.Bn + nd);
+                            const int m = (i * 2 + j) * 2 + kk;
+                            fe[0][m] = e3.x; fe[1][m] = e3.y; fe[2][m] = e3.z;
+                            fb[0][m] = b3.x; fb[1][m] = b3.y; fb[2][m] = b3.z;
+                        }
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    tri_coef(fe[d], g.E0[d], g.inv_vol, cE[d]);
+                    tri_coef(fb[d], g.B0[d], g.inv_vol, cB[d]);
+                }
+            }
 
-            // pass A: stayers of this run (particles of segment a still in cell a)
-            const int lb = pst - plo;
+            // pass A: stayers of each run (particles of segment a still in cell a; NaN: no)
             int my_stay = 0;
-            if (active && MODE != MODE_DEPOSIT) {
-                for (int jj = 0; jj < len; ++jj) {
-                    const float ox = sP[0][lb + jj], oy = sP[1][lb + jj], oz = sP[2][lb + jj];
-                    if (ox == ox && dcell(ox) == 0 && dcell(oy) == 0 && dcell(oz) == 0) ++my_stay;
-                }
+            if (MODE != MODE_DEPOSIT) {
+#pragma unroll
+                for (int jj = 0; jj < KRUN; ++jj)
+                    if (in_cell(rx[jj], ry[jj], rz[jj])) ++my_stay;
             }
-            int tot_stay;
-            const int sex = block_exscan(my_stay, s_w, tot_stay);
-            s_scan[tid] = sex;
-            __syncthreads();
-            int base = 0;
-            if (active && MODE != MODE_DEPOSIT) {
-                const int ft = max(s_rb[c], gq) - gq;
-                base = s_carry[c] + sex - s_scan[ft];
-            }
-            __syncthreads();
-            if (active && MODE != MODE_DEPOSIT) {
-                if (q + 1 == s_rb[c + 1] || tid == NT - 1 || q + 1 == R) s_carry[c] = base + my_stay;
-            }
+            const int sincl = wave_incl_scan(my_stay, lane);
+            const int sex = sincl - my_stay;
+            const int sfirst = __shfl(sex, R[k], WS);
+            const int base = (k == 0 ? carry : 0) + sex - sfirst;
 
-            // pass B: process the run
-            if (active) {
-                const int lx = c / (g.T[1] * g.T[2]), ly = (c / g.T[2]) % g.T[1], lz = c % g.T[2];
-                const int ax = x0 + lx, ay = y0 + ly, az = z0 + lz;
-                float4 cE[8], cB[8];
-                if (MODE == MODE_PUSH) {
+            float mom[3][8];
 #pragma unroll
-                    for (int i = 0; i < 2; ++i)
-#pragma unroll
-                        for (int j = 0; j < 2; ++j)
-#pragma unroll
-                            for (int k = 0; k < 2; ++k) {
-                                const int nd = nid_of(g, wrapc(ax + i, g.n[0]), wrapc(ay + j, g.n[1]),
-                                                      wrapc(az + k, g.n[2]));
-                                cE[(i * 2 + j) * 2 + k] = A.E[nd];
-                                cB[(i * 2 + j) * 2 + k] = A.Bn[nd];
-                            }
-                }
-                float acc[8][3];
-#pragma unroll
-                for (int m = 0; m < 8; ++m) acc[m][0] = acc[m][1] = acc[m][2] = 0.0f;
-                bool acc_used = false;
-                int run_cnt = 0, run_stay = 0, rank = 0;
-                const int a_start = (MODE != MODE_DEPOSIT) ? S.start[cbase + c] : 0;
-                const int a_end = (MODE != MODE_DEPOSIT) ? S.start[cbase + c + 1] : 0;
-                const float qom = S.qom, qs = S.qs;
+            for (int m = 0; m < 8; ++m) mom[0][m] = mom[1][m] = mom[2][m] = 0.0f;
+            int run_in = 0, rank = 0, qn = 0, kn = 0;
+            const int a_start = (MODE != MODE_DEPOSIT && active) ? S.start[cbase + c] : 0;
+            const int a_end = (MODE != MODE_DEPOSIT && active) ? S.start[cbase + c + 1] : 0;
 
-                for (int jj = 0; jj < len; ++jj) {
-                    const int i = lb + jj;
-                    const float ox = sP[0][i], oy = sP[1][i], oz = sP[2][i];
-                    if (!(ox == ox)) continue;  // slot of a particle removed by the sink
-                    float vx = sP[3][i], vy = sP[4][i], vz = sP[5][i];
-                    const int dbx = dcell(ox), dby = dcell(oy), dbz = dcell(oz);
-                    const bool moved = (dbx | dby | dbz) != 0;
-                    float px = ox - (float)dbx, py = oy - (float)dby, pz = oz - (float)dbz;
-                    const int bx = ax + dbx, by = ay + dby, bz = az + dbz;  // unwrapped
-
-                    if (MODE == MODE_PUSH) {
-                        float wt[8];
-                        tri_weights(px, py, pz, wt);
-                        float Ex = 0, Ey = 0, Ez = 0, Bx = 0, By = 0, Bz = 0;
-                        if (!moved) {
+            // pass B, fast path
 #pragma unroll
-                            for (int m = 0; m < 8; ++m) {
-                                Ex += cE[m].x * wt[m]; Ey += cE[m].y * wt[m]; Ez += cE[m].z * wt[m];
-                                Bx += cB[m].x * wt[m]; By += cB[m].y * wt[m]; Bz += cB[m].z * wt[m];
-                            }
-                        } else {
-                            const int bxw = wrapc(bx, g.n[0]), byw = wrapc(by, g.n[1]), bzw = wrapc(bz, g.n[2]);
-#pragma unroll
-                            for (int ii = 0; ii < 2; ++ii)
-#pragma unroll
-                                for (int jj2 = 0; jj2 < 2; ++jj2)
-#pragma unroll
-                                    for (int kk = 0; kk < 2; ++kk) {
-                                        const int nd = nid_of(g, wrapc(bxw + ii, g.n[0]), wrapc(byw + jj2, g.n[1]),
-                                                              wrapc(bzw + kk, g.n[2]));
-                                        const float4 e4 = A.E[nd], b4 = A.Bn[nd];
-                                        const float ww = wt[(ii * 2 + jj2) * 2 + kk];
-                                        Ex += e4.x * ww; Ey += e4.y * ww; Ez += e4.z * ww;
-                                        Bx += b4.x * ww; By += b4.y * ww; Bz += b4.z * ww;
-                                    }
-                        }
-                        // background uniform part (weights sum to one), then the division by the
-                        // cell volume of trilinearInterpolationF2P (cell.h:884)
-                        Ex += g.E0[0]; Ey += g.E0[1]; Ez += g.E0[2];
-                        Bx += g.B0[0]; By += g.B0[1]; Bz += g.B0[2];
-                        Ex *= g.inv_vol; Ey *= g.inv_vol; Ez *= g.inv_vol;
-                        Bx *= g.inv_vol; By *= g.inv_vol; Bz *= g.inv_vol;
-                        // adaptive sub-cycling (cell.h:948-952)
-                        const float Bmag = Bx * Bx + By * By + Bz * Bz;
-                        const float dt_sub0 = g.pi_c / (4.0f * fabsf(qom) * Bmag);
-                        const float ratio = g.dt / dt_sub0;
-                        const int nsub = (ratio < 99.0f) ? (int)ratio + 1 : 100;
-                        const float dts = g.dt / (float)nsub;
-                        const float kq = qom * 0.5f * dts;
-                        const float sx = dts * g.inv_w[0], sy = dts * g.inv_w[1], sz = dts * g.inv_w[2];
-                        for (int sc = 0; sc < nsub; ++sc) {
-                            // Boris rotation (particle.h:31-49)
-                            const float tx_ = kq * Bx, ty_ = kq * By, tz_ = kq * Bz;
-                            const float t2 = tx_ * tx_ + ty_ * ty_ + tz_ * tz_;
-                            const float f = 2.0f / (1.0f + t2);
-                            const float sx_ = f * tx_, sy_ = f * ty_, sz_ = f * tz_;
-                            const float mx = vx + kq * Ex, my = vy + kq * Ey, mz = vz + kq * Ez;
-                            const float qx = mx + (my * tz_ - mz * ty_);
-                            const float qy = my + (mz * tx_ - mx * tz_);
-                            const float qz = mz + (mx * ty_ - my * tx_);
-                            vx = mx + (qy * sz_ - qz * sy_) + kq * Ex;
-                            vy = my + (qz * sx_ - qx * sz_) + kq * Ey;
-                            vz = mz + (qx * sy_ - qy * sx_) + kq * Ez;
-                            // updatePosition (particle.h:24-26) in cell units
-                            px += vx * sx;
-                            py += vy * sy;
-                            pz += vz * sz;
-                        }
-                    }
-
-                    // cell after the move (relative to b) and offsets within it
-                    int ex = bx, ey = by, ez = bz;
-                    float qx_ = px, qy_ = py, qz_ = pz;
-                    bool sunk = false;
-                    if (MODE == MODE_PUSH) {
-                        const int dex = dcell(px), dey = dcell(py), dez = dcell(pz);
-                        ex += dex; ey += dey; ez += dez;
-                        qx_ -= (float)dex; qy_ -= (float)dey; qz_ -= (float)dez;
-                        if (g.sink) {
-                            const float ax_ = g.origin[0] + ((float)wrapc(ex, g.n[0]) + qx_) * g.w[0] - g.sink_c[0];
-                            const float ay_ = g.origin[1] + ((float)wrapc(ey, g.n[1]) + qy_) * g.w[1] - g.sink_c[1];
-                            const float az_ = g.origin[2] + ((float)wrapc(ez, g.n[2]) + qz_) * g.w[2] - g.sink_c[2];
-                            sunk = (ax_ * ax_ + ay_ * ay_ + az_ * az_) <= g.sink_r2;
-                        }
-                    }
-
-                    if (MODE != MODE_DEPOSIT) {
-                        int oidx, olim;
-                        if (!moved) {
-                            oidx = a_start + base + rank;
-                            olim = a_end;
-                            ++rank;
-                        } else {
-                            const int bc = cid_of(g, wrapc(bx, g.n[0]), wrapc(by, g.n[1]), wrapc(bz, g.n[2]));
-                            oidx = S.start[bc] + S.stay[bc] + atomicAdd(&S.arr[bc], 1);
-                            olim = S.start[bc + 1];
-                        }
-                        // guard: a slot outside the cell's range means the per-cell counts are
-                        // inconsistent; report instead of writing out of bounds
-                        if (oidx < 0 || oidx >= olim || (long long)oidx >= S.cap) {
-                            atomicOr(A.err, 1);
-                            continue;
-                        }
-                        float* o = S.out;
-                        o[oidx] = sunk ? NaNf : px;
-                        o[S.cap + oidx] = py;
-                        o[2 * S.cap + oidx] = pz;
-                        o[3 * S.cap + oidx] = vx;
-                        o[4 * S.cap + oidx] = vy;
-                        o[5 * S.cap + oidx] = vz;
-                        if (TRACK) S.id_out[oidx] = sId[i];
+            for (int jj = 0; jj < KRUN; ++jj) {
+                int qe = -1;       // arrival queue entry of this lane
+                bool leaver = false;
+                float px = rx[jj], py = ry[jj], pz = rz[jj];
+                float vx = rvx[jj], vy = rvy[jj], vz = rvz[jj];
+                if (alive(px, py, pz)) {
+                    if (!in_cell(px, py, pz)) {
+                        qe = (pst + jj - plo) | (k << 9);  // sits in another cell
+                    } else {
+                        bool sunk = false;
                         if (MODE == MODE_PUSH) {
-                            if (sunk) {
-                                atomicAdd(A.removed, 1ull);
-                            } else {
-                                const bool e_is_b = (ex == bx) && (ey == by) && (ez == bz);
-                                if (!moved && e_is_b) {
-                                    ++run_cnt;
-                                    ++run_stay;
-                                } else {
-                                    count_add(g, s_cnt, S.cnt_acc, x0, y0, z0, ex, ey, ez);
-                                    if (e_is_b) count_add(g, s_stay, S.stay_acc, x0, y0, z0, bx, by, bz);
-                                }
+                            const float Ex = tri_eval(cE[0], px, py, pz), Ey = tri_eval(cE[1], px, py, pz),
+                                        Ez = tri_eval(cE[2], px, py, pz);
+                            const float Bx = tri_eval(cB[0], px, py, pz), By = tri_eval(cB[1], px, py, pz),
+                                        Bz = tri_eval(cB[2], px, py, pz);
+                            boris_move(g, qom, subk, Ex, Ey, Ez, Bx, By, Bz, vx, vy, vz, px, py, pz);
+                            sunk = !finite3(px, py, pz);
+                            if (g.sink && !sunk) {
+                                const int dex = dcell(px), dey = dcell(py), dez = dcell(pz);
+                                const float sx_ = g.origin[0] + ((float)wrapc(ax + dex, g.n[0]) + px - (float)dex) * g.w[0] - g.sink_c[0];
+                                const float sy_ = g.origin[1] + ((float)wrapc(ay + dey, g.n[1]) + py - (float)dey) * g.w[1] - g.sink_c[1];
+                                const float sz_ = g.origin[2] + ((float)wrapc(az + dez, g.n[2]) + pz - (float)dez) * g.w[2] - g.sink_c[2];
+                                sunk = (sx_ * sx_ + sy_ * sy_ + sz_ * sz_) <= g.sink_r2;
                             }
                         }
-                    }
-
-                    if (MODE != MODE_REBIN && !sunk) {
-                        float wt[8];
-                        tri_weights(qx_, qy_, qz_, wt);
-                        const float jx = qs * vx, jy = qs * vy, jz = qs * vz;
-                        if (ex == ax && ey == ay && ez == az) {
-#pragma unroll
-                            for (int m = 0; m < 8; ++m) {
-                                acc[m][0] += jx * wt[m];
-                                acc[m][1] += jy * wt[m];
-                                acc[m][2] += jz * wt[m];
+                        if (MODE != MODE_DEPOSIT) {
+                            const int oidx = a_start + base + rank;
+                            ++rank;
+                            if (oidx < 0 || oidx >= a_end || (long long)oidx >= S.cap) {
+                                atomicOr(A.err, 1);
+                            } else {
+                                store_particle(S, oidx, sunk ? NaNf : px, py, pz, vx, vy, vz);
+                                if (TRACK) S.id_out[oidx] = rid[jj];
                             }
-                            acc_used = true;
-                        } else {
-                            deposit_direct(A, sJ, x0, y0, z0, ex - x0, ey - y0, ez - z0, wt, jx, jy, jz);
+                        }
+                        if (MODE == MODE_PUSH && sunk) {
+                            atomicAdd(A.removed, 1ull);
+                        } else if (MODE != MODE_REBIN) {
+                            if (MODE == MODE_DEPOSIT || in_cell(px, py, pz)) {
+                                const float jx = qs * vx, jy = qs * vy, jz = qs * vz;
+                                const float xy = px * py, xz = px * pz, yz = py * pz, xyz = xy * pz;
+                                const float mono[8] = {1.0f, px, py, pz, xy, xz, yz, xyz};
+#pragma unroll
+                                for (int m = 0; m < 8; ++m) {
+                                    mom[0][m] = fmaf(jx, mono[m], mom[0][m]);
+                                    mom[1][m] = fmaf(jy, mono[m], mom[1][m]);
+                                    mom[2][m] = fmaf(jz, mono[m], mom[2][m]);
+                                }
+                                ++run_in;
+                            } else {
+                                leaver = true;  // leaves cell a: deposit and count after the loop
+                            }
                         }
                     }
                 }
-                if (MODE != MODE_REBIN && acc_used) {
+                // ballot compaction of both queues (every lane executes this)
+                const unsigned long long bq = __ballot(qe >= 0);
+                if (qe >= 0) Q[qn + ballot_rank(bq)] = (unsigned short)qe;
+                qn += __popcll(bq);
+                const unsigned long long bk = __ballot(leaver);
+                if (leaver) {
+                    const int slot = kn + ballot_rank(bk);
+                    if (slot < K0CAP) {
+                        float* e = K0 + slot * 8;
+                        e[0] = px; e[1] = py; e[2] = pz; e[3] = vx; e[4] = vy; e[5] = vz;
+                        e[6] = __int_as_float(k);
+                    } else {
+                        // buffer full: handle in place
+                        const int dex = dcell(px), dey = dcell(py), dez = dcell(pz);
+                        float wt[8];
+                        tri_weights(px - (float)dex, py - (float)dey, pz - (float)dez, wt);
+                        deposit_direct(A, sJ, x0, y0, z0, lx + dex, ly + dey, lz + dez, wt, qs * vx, qs * vy, qs * vz);
+                        count_add(g, s_cnt, S.cnt_acc, x0, y0, z0, ax + dex, ay + dey, az + dez);
+                    }
+                }
+                kn += __popcll(bk);
+            }
+            if (MODE != MODE_REBIN && run_in > 0) {
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    float N[8];
+                    mom_to_nodes(mom[d], N);
 #pragma unroll
                     for (int i = 0; i < 2; ++i)
 #pragma unroll
                         for (int j = 0; j < 2; ++j)
 #pragma unroll
-                            for (int k = 0; k < 2; ++k) {
-                                const int l = ((lx + 1 + i) * g.nj[1] + (ly + 1 + j)) * g.nj[2] + (lz + 1 + k);
-                                const int m = (i * 2 + j) * 2 + k;
-                                lds_add(&sJ[l], acc[m][0]);
-                                lds_add(&sJ[g.NJ + l], acc[m][1]);
-                                lds_add(&sJ[2 * g.NJ + l], acc[m][2]);
+                            for (int kk = 0; kk < 2; ++kk) {
+                                const int l = ((lx + 1 + i) * g.nj[1] + (ly + 1 + j)) * g.nj[2] + (lz + 1 + kk);
+                                fix_add(&sJ[d * g.NJ + l], N[(i * 2 + j) * 2 + kk], g.fix_pre);
                             }
                 }
-                if (MODE == MODE_PUSH && (run_cnt | run_stay)) {
-                    atomicAdd(&s_cnt[c], run_cnt);
-                    atomicAdd(&s_stay[c], run_stay);
+            }
+            if (MODE == MODE_PUSH && run_in > 0) {
+                atomicAdd(&s_cnt[c], run_in);
+                atomicAdd(&s_stay[c], run_in);
+            }
+            wave_sync();
+
+            // leavers: deposit and count at the new cell, one per lane
+            const int kq_n = min(kn, K0CAP);
+            for (int qi = lane; qi < kq_n; qi += WS) {
+                const float* e = K0 + qi * 8;
+                const float px = e[0], py = e[1], pz = e[2], vx = e[3], vy = e[4], vz = e[5];
+                const int kc = c_cur + __float_as_int(e[6]);
+                const int qlx = fdivi(kc, g.inv_T12);
+                const int qr = kc - qlx * g.T[1] * g.T[2];
+                const int qly = fdivi(qr, g.inv_T[2]);
+                const int qlz = qr - qly * g.T[2];
+                const int dex = dcell(px), dey = dcell(py), dez = dcell(pz);
+                float wt[8];
+                tri_weights(px - (float)dex, py - (float)dey, pz - (float)dez, wt);
+                deposit_direct(A, sJ, x0, y0, z0, qlx + dex, qly + dey, qlz + dez, wt, qs * vx, qs * vy, qs * vz);
+                count_add(g, s_cnt, S.cnt_acc, x0, y0, z0, x0 + qlx + dex, y0 + qly + dey, z0 + qlz + dez);
+            }
+            // particles sitting in another cell b: full processing, one per lane
+            for (int qi = lane; qi < qn; qi += WS) {
+                const int ent = Q[qi];
+                const int inq = plo + (ent & 511);
+                const int kc = c_cur + (ent >> 9);
+                const int qlx = fdivi(kc, g.inv_T12);
+                const int qr = kc - qlx * g.T[1] * g.T[2];
+                const int qly = fdivi(qr, g.inv_T[2]);
+                const int qlz = qr - qly * g.T[2];
+                const float ox = S.in[0][inq], oy = S.in[1][inq], oz = S.in[2][inq];
+                float vx = S.in[3][inq], vy = S.in[4][inq], vz = S.in[5][inq];
+                const int dbx = dcell(ox), dby = dcell(oy), dbz = dcell(oz);
+                float px = ox - (float)dbx, py = oy - (float)dby, pz = oz - (float)dbz;
+                const int bx = x0 + qlx + dbx, by = y0 + qly + dby, bz = z0 + qlz + dbz;  // unwrapped
+                const int bxw = wrapc(bx, g.n[0]), byw = wrapc(by, g.n[1]), bzw = wrapc(bz, g.n[2]);
+                bool sunk = false;
+                int dex = 0, dey = 0, dez = 0;
+                if (MODE == MODE_PUSH) {
+                    float Ex, Ey, Ez, Bx, By, Bz;
+                    gather_direct(A, bxw, byw, bzw, px, py, pz, Ex, Ey, Ez, Bx, By, Bz);
+                    boris_move(g, qom, subk, Ex, Ey, Ez, Bx, By, Bz, vx, vy, vz, px, py, pz);
+                    sunk = !finite3(px, py, pz);
+                    if (!sunk) {
+                        dex = dcell(px); dey = dcell(py); dez = dcell(pz);
+                    }
+                    if (g.sink && !sunk) {
+                        const float rx_ = g.origin[0] + ((float)wrapc(bx + dex, g.n[0]) + px - (float)dex) * g.w[0] - g.sink_c[0];
+                        const float ry_ = g.origin[1] + ((float)wrapc(by + dey, g.n[1]) + py - (float)dey) * g.w[1] - g.sink_c[1];
+                        const float rz_ = g.origin[2] + ((float)wrapc(bz + dez, g.n[2]) + pz - (float)dez) * g.w[2] - g.sink_c[2];
+                        sunk = (rx_ * rx_ + ry_ * ry_ + rz_ * rz_) <= g.sink_r2;
+                    }
+                }
+                bool ok = true;
+                if (MODE != MODE_DEPOSIT) {
+                    const int bc = cid_of(g, bxw, byw, bzw);
+                    const int oidx = S.start[bc] + S.stay[bc] + atomicAdd(&S.arr[bc], 1);
+                    if (oidx < 0 || oidx >= S.start[bc + 1] || (long long)oidx >= S.cap) {
+                        atomicOr(A.err, 1);
+                        ok = false;
+                    } else {
+                        store_particle(S, oidx, sunk ? NaNf : px, py, pz, vx, vy, vz);
+                        if (TRACK) S.id_out[oidx] = S.id_in[inq];
+                    }
+                }
+                if (MODE == MODE_PUSH && sunk) {
+                    atomicAdd(A.removed, 1ull);
+                } else if (MODE != MODE_REBIN && ok) {
+                    float wt[8];
+                    tri_weights(px - (float)dex, py - (float)dey, pz - (float)dez, wt);
+                    deposit_direct(A, sJ, x0, y0, z0, bx - x0 + dex, by - y0 + dey, bz - z0 + dez, wt, qs * vx,
+                                   qs * vy, qs * vz);
+                    if (MODE == MODE_PUSH) {
+                        count_add(g, s_cnt, S.cnt_acc, x0, y0, z0, bx + dex, by + dey, bz + dez);
+                        if ((dex | dey | dez) == 0) count_add(g, s_stay, S.stay_acc, x0, y0, z0, bx, by, bz);
+                    }
                 }
             }
-            __syncthreads();
+            // advance the wave's cursor past the runs of this chunk
+            const int klast = __shfl(k, nruns - 1, WS);
+            if (all_runs <= WS) {
+                c_cur = min(c_cur + WS, chi);
+                pos_cur = c_cur < chi ? s_seg[c_cur] : 0;
+                carry = 0;
+            } else if (R[klast + 1] == WS) {  // the last cell ended with run 63
+                c_cur = c_cur + klast + 1;
+                pos_cur = c_cur < chi ? s_seg[c_cur] : 0;
+                carry = 0;
+            } else {  // cell c_cur + klast continues in the next chunk
+                const int sl = __shfl(sincl, nruns - 1, WS);
+                const int s0 = __shfl(sex, R[klast], WS);
+                carry = (klast == 0 ? carry : 0) + sl - s0;
+                c_cur = c_cur + klast;
+                pos_cur = phi;
+            }
+            wave_sync();
         }
+        __syncthreads();
         if (MODE == MODE_PUSH) {
             for (int i = tid; i < g.TC; i += NT) {
                 if (s_cnt[i]) atomicAdd(&S.cnt_acc[cbase + i], s_cnt[i]);
                 if (s_stay[i]) atomicAdd(&S.stay_acc[cbase + i], s_stay[i]);
             }
         }
-        __syncthreads();
     }
+    __syncthreads();
     if (MODE != MODE_REBIN) {
         float* dst = A.jpart + (long long)t * 3 * g.NJ;
-        for (int i = tid; i < 3 * g.NJ; i += NT) dst[i] = sJ[i];
+        for (int i = tid; i < 3 * g.NJ; i += NT) dst[i] = (float)((double)sJ[i] * g.fix_inv);
     }
 }
 
@@ -1042,8 +1314,10 @@ PushArgs push_args(ipic_ctx* ctx) {
     for (int s = 0; s < A.nsp; ++s) {
         Species& S = ctx->sp[s];
         SpArgs& a = A.sp[s];
-        a.in = S.X[S.cur];
-        a.out = S.X[1 - S.cur];
+        for (int k = 0; k < 6; ++k) {
+            a.in[k] = S.X[S.cur] + (size_t)k * S.cap;
+            a.out[k] = S.X[1 - S.cur] + (size_t)k * S.cap;
+        }
         a.id_in = S.ids[S.cur];
         a.id_out = S.ids[1 - S.cur];
         a.cap = S.cap;
@@ -1055,6 +1329,7 @@ PushArgs push_args(ipic_ctx* ctx) {
         a.stay_acc = S.stay_acc;
         a.qom = (float)S.qom;
         a.qs = (float)(S.q / (8.0 * vol));
+        a.subk = (float)(4.0 * std::fabs(S.qom) * ctx->d.dt / (M_PI * ctx->d.c));
     }
     A.E = ctx->Eg ? ctx->Eg : ctx->E;
     A.Bn = ctx->Bn;
@@ -1101,14 +1376,15 @@ ScanArgs scan_args(ipic_ctx* ctx) {
 int launch_particles(ipic_ctx* ctx, int mode) {
     PushArgs A = push_args(ctx);
     dim3 grid(ctx->g.ntiles), block(NT);
+    const size_t lds = (size_t)particle_layout(ctx->g.NJ, ctx->g.TC).total;
     if (ctx->track) {
-        if (mode == MODE_PUSH) hipLaunchKernelGGL((k_particles<MODE_PUSH, true>), grid, block, 0, ctx->stream, A);
-        else if (mode == MODE_DEPOSIT) hipLaunchKernelGGL((k_particles<MODE_DEPOSIT, true>), grid, block, 0, ctx->stream, A);
-        else hipLaunchKernelGGL((k_particles<MODE_REBIN, true>), grid, block, 0, ctx->stream, A);
+        if (mode == MODE_PUSH) hipLaunchKernelGGL((k_particles<MODE_PUSH, true>), grid, block, lds, ctx->stream, A);
+        else if (mode == MODE_DEPOSIT) hipLaunchKernelGGL((k_particles<MODE_DEPOSIT, true>), grid, block, lds, ctx->stream, A);
+        else hipLaunchKernelGGL((k_particles<MODE_REBIN, true>), grid, block, lds, ctx->stream, A);
     } else {
-        if (mode == MODE_PUSH) hipLaunchKernelGGL((k_particles<MODE_PUSH, false>), grid, block, 0, ctx->stream, A);
-        else if (mode == MODE_DEPOSIT) hipLaunchKernelGGL((k_particles<MODE_DEPOSIT, false>), grid, block, 0, ctx->stream, A);
-        else hipLaunchKernelGGL((k_particles<MODE_REBIN, false>), grid, block, 0, ctx->stream, A);
+        if (mode == MODE_PUSH) hipLaunchKernelGGL((k_particles<MODE_PUSH, false>), grid, block, lds, ctx->stream, A);
+        else if (mode == MODE_DEPOSIT) hipLaunchKernelGGL((k_particles<MODE_DEPOSIT, false>), grid, block, lds, ctx->stream, A);
+        else hipLaunchKernelGGL((k_particles<MODE_REBIN, false>), grid, block, lds, ctx->stream, A);
     }
     HIPCK(ctx, hipGetLastError());
     if (mode != MODE_DEPOSIT)
@@ -1329,9 +1605,24 @@ int ipic_gpu_create(const ipic_desc* desc, ipic_ctx** out) {
     g.ncell = (int)ncell;
     g.NJ = g.nj[0] * g.nj[1] * g.nj[2];
     g.nmax = std::max(g.n[0], std::max(g.n[1], g.n[2]));
+    g.s0 = g.n[1] * g.n[2];
     g.dt = (float)d.dt;
     g.inv_vol = (float)(1.0 / (d.cell_width[0] * d.cell_width[1] * d.cell_width[2]));
     g.pi_c = (float)(M_PI * d.c);
+    for (int k = 0; k < 3; ++k) g.inv_T[k] = 1.0f / (float)g.T[k];
+    g.inv_T12 = 1.0f / (float)(g.T[1] * g.T[2]);
+    {
+        // fixed-point scale of the LDS J accumulator: one particle at speed c contributes about
+        // 2^30 units, leaving 2^33 of headroom for the sums of a tile node
+        const double vol = d.cell_width[0] * d.cell_width[1] * d.cell_width[2];
+        double qmax = 0.0;
+        for (int s = 0; s < d.nspecies; ++s) qmax = std::max(qmax, std::fabs(d.species[s].q) / (8.0 * vol));
+        int bits = 40;
+        if (qmax * d.c > 0.0 && std::isfinite(qmax * d.c)) bits = 30 - (int)std::ceil(std::log2(qmax * d.c));
+        bits = std::max(-60, std::min(120, bits));
+        g.fix_pre = (float)std::ldexp(1.0, bits - 32);
+        g.fix_inv = std::ldexp(1.0, -bits);
+    }
     g.sink = d.sink ? 1 : 0;
     g.sink_r2 = (float)(d.sink_radius * d.sink_radius);
 
