@@ -34,7 +34,7 @@ constexpr int WS = 64;           // wavefront size
 constexpr int NW = NT / WS;      // waves per block
 constexpr int KRUN = 7;          // max particles per run (one lane, held in registers)
 constexpr int WCH = WS * KRUN;   // max particles per wave chunk
-constexpr int K0CAP = 64;        // leaver buffer entries per wave chunk
+constexpr int K0CAP = 32;        // leaver buffer entries per wave chunk
 constexpr int TCMAX = 512;       // max cells per tile
 constexpr int NJMAX = 11 * 11 * 11;  // max J nodes per tile ((T+3)^3, T <= 8)
 constexpr int JPMAX = 4;         // max (tile, local) pairs per axis coordinate
@@ -70,8 +70,9 @@ struct Geo {
 };
 
 struct SpArgs {
-    const float* in[6];   // input arrays: offsets ox oy oz (cell units), velocities vx vy vz
-    float* out[6];        // output arrays, same order
+    const float* in;      // input arrays, `cap` floats each, back to back: offsets ox oy oz
+                          // (cell units), velocities vx vy vz
+    float* out;           // output arrays, same layout
     const uint32_t* id_in;
     uint32_t* id_out;
     long long cap;
@@ -164,16 +165,16 @@ __device__ __forceinline__ int block_exscan(int v, int* s_w, int& total) {
 
 // J is accumulated in LDS as 64-bit fixed point (fix_bits fractional bits): integer atomics
 // are fast where float atomics on shared addresses serialise, and the sum is exact and
-// independent of the order of the adds.  |v| * 2^fix_bits = hi * 2^32 + lo with
-// a = |v| * 2^(fix_bits - 32), hi = floor(a) and lo = (a - hi) * 2^32 rounded to nearest:
-// a - hi is exact for a >= 0 and at most 1 - 2^-24, so lo < 2^32; the sign is applied to the
-// integer, which keeps the rounding symmetric (unbiased).
+// independent of the order of the adds.  With a = v * 2^(fix_bits - 32) (clamped to 2^30),
+// v * 2^fix_bits = hi * 2^32 + lo where hi = floor(a) is the signed high word and
+// lo = (a - hi) * 2^32, rounded to nearest, the unsigned low word: the two words are the
+// two's-complement 64-bit integer.  a - hi is exact except for -1 < a < 0, where it is
+// rounded to nearest; both roundings are unbiased, so sums over many particles keep no drift.
 __device__ __forceinline__ void fix_add(long long* p, float v, float pre) {
-    const float a = fminf(fabsf(v) * pre, 1073741823.0f);
+    const float a = __builtin_amdgcn_fmed3f(v * pre, -1073741824.0f, 1073741824.0f);
     const float h = floorf(a);
-    const unsigned lo = (unsigned)rintf((a - h) * 4294967296.0f);
-    long long q = (long long)(((unsigned long long)(unsigned)h << 32) + lo);
-    if (v < 0.0f) q = -q;
+    const unsigned lo = (unsigned)rintf(fminf((a - h) * 4294967296.0f, 4294967040.0f));
+    const long long q = (long long)(((unsigned long long)(unsigned)(int)h << 32) | lo);
     __hip_atomic_fetch_add(p, q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
@@ -382,24 +383,68 @@ __device__ __forceinline__ bool in_cell(float x, float y, float z) {
     return x >= 0.0f && x <= 1.0f && y >= 0.0f && y <= 1.0f && z >= 0.0f && z <= 1.0f;
 }
 
-__device__ __forceinline__ void store_particle(const SpArgs& S, int oidx, float px, float py, float pz, float vx,
-                                               float vy, float vz) {
-    S.out[0][oidx] = px;
-    S.out[1][oidx] = py;
-    S.out[2][oidx] = pz;
-    S.out[3][oidx] = vx;
-    S.out[4][oidx] = vy;
-    S.out[5][oidx] = vz;
+// Particle arrays are read and written through buffer resources: one resource spans the six
+// arrays of a species (offset from the array start in `voff`, array d at byte d * cs), so a
+// lane keeps one 32-bit offset for all six and an out-of-range access reads 0 / is dropped.
+typedef __amdgpu_buffer_rsrc_t Rsrc;
+
+__device__ __forceinline__ Rsrc make_rsrc(const void* p, long long bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, (int)(unsigned)bytes, 0x00020000);
 }
 
-// Dynamic LDS layout of k_particles (bytes, 8-aligned start)
+__device__ __forceinline__ float ldp(Rsrc r, unsigned voff, unsigned soff) {
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+
+__device__ __forceinline__ void stp(Rsrc r, float v, unsigned voff, unsigned soff) {
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, voff, soff, 0);
+}
+
+__device__ __forceinline__ void store_particle(Rsrc r, unsigned cs, int oidx, float px, float py, float pz,
+                                               float vx, float vy, float vz) {
+    const unsigned vo = (unsigned)oidx * 4u;
+    stp(r, px, vo, 0);
+    stp(r, py, vo, cs);
+    stp(r, pz, vo, 2 * cs);
+    stp(r, vx, vo, 3 * cs);
+    stp(r, vy, vo, 4 * cs);
+    stp(r, vz, vo, 5 * cs);
+}
+
+// E and B at offsets (x,y,z) of a cell from the tile's node field in LDS: F4[n] = (Ex,Ey,Ez,Bx),
+// F2[n] = (By,Bz) at node n, background included and divided by the cell volume; nb is the
+// cell's (0,0,0) corner, sx and sy the node strides of x and y.
+__device__ __forceinline__ void gather_lds(const float4* F4, const float2* F2, int nb, int sx, int sy, float x,
+                                           float y, float z, float& Ex, float& Ey, float& Ez, float& Bx,
+                                           float& By, float& Bz) {
+    float wt[8];
+    tri_weights(x, y, z, wt);
+    Ex = Ey = Ez = Bx = By = Bz = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                const int n = nb + i * sx + j * sy + k;
+                const float4 a = F4[n];
+                const float2 b = F2[n];
+                const float w = wt[(i * 2 + j) * 2 + k];
+                Ex = fmaf(a.x, w, Ex); Ey = fmaf(a.y, w, Ey); Ez = fmaf(a.z, w, Ez);
+                Bx = fmaf(a.w, w, Bx); By = fmaf(b.x, w, By); Bz = fmaf(b.y, w, Bz);
+            }
+}
+
+// Dynamic LDS layout of k_particles (bytes, 16-aligned sections)
 struct PLayout {
-    int off_seg, off_cnt, off_stay, off_wlo, off_wave, wave_bytes, total;
+    int off_f4, off_f2, off_seg, off_cnt, off_stay, off_wlo, off_wave, wave_bytes, total;
 };
 
-__host__ __device__ inline PLayout particle_layout(int NJ, int TC) {
+__host__ __device__ inline PLayout particle_layout(int NJ, int TC, int NF) {
     PLayout L;
     int o = 3 * NJ * 8;             // J, int64
+    L.off_f4 = o; o += NF * 16;     // node field (Ex,Ey,Ez,Bx)
+    L.off_f2 = o; o += NF * 8;      // node field (By,Bz)
     L.off_seg = o; o += (TC + 1) * 4;
     L.off_cnt = o; o += TC * 4;
     L.off_stay = o; o += TC * 4;
@@ -414,12 +459,16 @@ __host__ __device__ inline PLayout particle_layout(int NJ, int TC) {
     return L;
 }
 
+__host__ __device__ inline int field_nodes(const Geo& g) { return (g.T[0] + 1) * (g.T[1] + 1) * (g.T[2] + 1); }
+
 template <int MODE, bool TRACK>
 __global__ __launch_bounds__(NT) void k_particles(PushArgs A) {
     const Geo& g = A.g;
     extern __shared__ __align__(16) unsigned char smem[];
-    const PLayout L = particle_layout(g.NJ, g.TC);
+    const PLayout L = particle_layout(g.NJ, g.TC, MODE == MODE_PUSH ? field_nodes(g) : 0);
     long long* sJ = (long long*)smem;
+    float4* sF4 = (float4*)(smem + L.off_f4);
+    float2* sF2 = (float2*)(smem + L.off_f2);
     int* s_seg = (int*)(smem + L.off_seg);
     int* s_cnt = (int*)(smem + L.off_cnt);
     int* s_stay = (int*)(smem + L.off_stay);
@@ -436,12 +485,28 @@ __global__ __launch_bounds__(NT) void k_particles(PushArgs A) {
     const int x0 = tx * g.T[0], y0 = ty * g.T[1], z0 = tz * g.T[2];
     const int cbase = t * g.TC;
     const float NaNf = __builtin_nanf("");
+    const int fsy = g.T[2] + 1, fsx = (g.T[1] + 1) * fsy;  // node strides of the LDS field
 
     if (MODE != MODE_REBIN)
         for (int i = tid; i < 3 * g.NJ; i += NT) sJ[i] = 0;
+    if (MODE == MODE_PUSH) {
+        // the tile's (T+1)^3 nodes of E and B, background added, divided by the cell volume
+        const int nf = (g.T[0] + 1) * fsx;
+        for (int i = tid; i < nf; i += NT) {
+            const int ix = i / fsx, r = i - ix * fsx, iy = r / fsy, iz = r - iy * fsy;
+            const int nd = wrap1(x0 + ix, g.n[0]) * g.s0 + wrap1(y0 + iy, g.n[1]) * g.n[2] + wrap1(z0 + iz, g.n[2]);
+            const float3 e = ld3(A.E + nd), b = ld3(A.Bn + nd);
+            const float iv = g.inv_vol;
+            sF4[i] = make_float4((e.x + g.E0[0]) * iv, (e.y + g.E0[1]) * iv, (e.z + g.E0[2]) * iv, (b.x + g.B0[0]) * iv);
+            sF2[i] = make_float2((b.y + g.B0[1]) * iv, (b.z + g.B0[2]) * iv);
+        }
+    }
 
     for (int s = 0; s < A.nsp; ++s) {
         const SpArgs& S = A.sp[s];
+        const Rsrc rin = make_rsrc(S.in, S.cap * 24);
+        const Rsrc rout = make_rsrc(S.out, S.cap * 24);
+        const unsigned cs = (unsigned)S.cap * 4u;  // byte stride between the six arrays
         __syncthreads();
         for (int i = tid; i <= g.TC; i += NT) s_seg[i] = S.seg[cbase + i];
         for (int i = tid; i < g.TC; i += NT) {
@@ -512,17 +577,18 @@ __global__ __launch_bounds__(NT) void k_particles(PushArgs A) {
             // this lane's run into registers
             float rx[KRUN], ry[KRUN], rz[KRUN], rvx[KRUN], rvy[KRUN], rvz[KRUN];
             uint32_t rid[TRACK ? KRUN : 1];
+            const unsigned vrun = (unsigned)pst * 4u;
 #pragma unroll
             for (int jj = 0; jj < KRUN; ++jj) {
                 if (jj < len) {
-                    const int q = pst + jj;
-                    rx[jj] = S.in[0][q];
-                    ry[jj] = S.in[1][q];
-                    rz[jj] = S.in[2][q];
-                    rvx[jj] = S.in[3][q];
-                    rvy[jj] = S.in[4][q];
-                    rvz[jj] = S.in[5][q];
-                    if (TRACK) rid[jj] = S.id_in[q];
+                    const unsigned vo = vrun + 4u * jj;
+                    rx[jj] = ldp(rin, vo, 0);
+                    ry[jj] = ldp(rin, vo, cs);
+                    rz[jj] = ldp(rin, vo, 2 * cs);
+                    rvx[jj] = ldp(rin, vo, 3 * cs);
+                    rvy[jj] = ldp(rin, vo, 4 * cs);
+                    rvz[jj] = ldp(rin, vo, 5 * cs);
+                    if (TRACK) rid[jj] = S.id_in[pst + jj];
                 } else {
                     rx[jj] = ry[jj] = rz[jj] = NaNf;
                     rvx[jj] = rvy[jj] = rvz[jj] = 0.0f;
@@ -535,30 +601,7 @@ __global__ __launch_bounds__(NT) void k_particles(PushArgs A) {
             const int ly = fdivi(crem, g.inv_T[2]);
             const int lz = crem - ly * g.T[2];
             const int ax = x0 + lx, ay = y0 + ly, az = z0 + lz;
-            float cE[3][8], cB[3][8];
-            if (MODE == MODE_PUSH && active) {
-                float fe[3][8], fb[3][8];
-                const int xs[2] = {ax * g.s0, wrap1(ax + 1, g.n[0]) * g.s0};
-                const int ys[2] = {ay * g.n[2], wrap1(ay + 1, g.n[1]) * g.n[2]};
-                const int zs[2] = {az, wrap1(az + 1, g.n[2])};
-#pragma unroll
-                for (int i = 0; i < 2; ++i)
-#pragma unroll
-                    for (int j = 0; j < 2; ++j)
-#pragma unroll
-                        for (int kk = 0; kk < 2; ++kk) {
-                            const int nd = xs[i] + ys[j] + zs[kk];
-                            const float3 e3 = ld3(A.E + nd), b3 = ld3(A.Bn + nd);
-                            const int m = (i * 2 + j) * 2 + kk;
-                            fe[0][m] = e3.x; fe[1][m] = e3.y; fe[2][m] = e3.z;
-                            fb[0][m] = b3.x; fb[1][m] = b3.y; fb[2][m] = b3.z;
-                        }
-#pragma unroll
-                for (int d = 0; d < 3; ++d) {
-                    tri_coef(fe[d], g.E0[d], g.inv_vol, cE[d]);
-                    tri_coef(fb[d], g.B0[d], g.inv_vol, cB[d]);
-                }
-            }
+            const int nb = lx * fsx + ly * fsy + lz;  // the run cell's first node in the LDS field
 
             // pass A: stayers of each run (particles of segment a still in cell a; NaN: no)
             int my_stay = 0;
@@ -592,10 +635,8 @@ __global__ __launch_bounds__(NT) void k_particles(PushArgs A) {
                     } else {
                         bool sunk = false;
                         if (MODE == MODE_PUSH) {
-                            const float Ex = tri_eval(cE[0], px, py, pz), Ey = tri_eval(cE[1], px, py, pz),
-                                        Ez = tri_eval(cE[2], px, py, pz);
-                            const float Bx = tri_eval(cB[0], px, py, pz), By = tri_eval(cB[1], px, py, pz),
-                                        Bz = tri_eval(cB[2], px, py, pz);
+                            float Ex, Ey, Ez, Bx, By, Bz;
+                            gather_lds(sF4, sF2, nb, fsx, fsy, px, py, pz, Ex, Ey, Ez, Bx, By, Bz);
                             boris_move(g, qom, subk, Ex, Ey, Ez, Bx, By, Bz, vx, vy, vz, px, py, pz);
                             sunk = !finite3(px, py, pz);
                             if (g.sink && !sunk) {
@@ -612,7 +653,7 @@ __global__ __launch_bounds__(NT) void k_particles(PushArgs A) {
                             if (oidx < 0 || oidx >= a_end || (long long)oidx >= S.cap) {
                                 atomicOr(A.err, 1);
                             } else {
-                                store_particle(S, oidx, sunk ? NaNf : px, py, pz, vx, vy, vz);
+                                store_particle(rout, cs, oidx, sunk ? NaNf : px, py, pz, vx, vy, vz);
                                 if (TRACK) S.id_out[oidx] = rid[jj];
                             }
                         }
@@ -705,17 +746,23 @@ __global__ __launch_bounds__(NT) void k_particles(PushArgs A) {
                 const int qr = kc - qlx * g.T[1] * g.T[2];
                 const int qly = fdivi(qr, g.inv_T[2]);
                 const int qlz = qr - qly * g.T[2];
-                const float ox = S.in[0][inq], oy = S.in[1][inq], oz = S.in[2][inq];
-                float vx = S.in[3][inq], vy = S.in[4][inq], vz = S.in[5][inq];
+                const unsigned vq = (unsigned)inq * 4u;
+                const float ox = ldp(rin, vq, 0), oy = ldp(rin, vq, cs), oz = ldp(rin, vq, 2 * cs);
+                float vx = ldp(rin, vq, 3 * cs), vy = ldp(rin, vq, 4 * cs), vz = ldp(rin, vq, 5 * cs);
                 const int dbx = dcell(ox), dby = dcell(oy), dbz = dcell(oz);
                 float px = ox - (float)dbx, py = oy - (float)dby, pz = oz - (float)dbz;
-                const int bx = x0 + qlx + dbx, by = y0 + qly + dby, bz = z0 + qlz + dbz;  // unwrapped
+                const int blx = qlx + dbx, bly = qly + dby, blz = qlz + dbz;  // tile-local cell b
+                const int bx = x0 + blx, by = y0 + bly, bz = z0 + blz;        // unwrapped
                 const int bxw = wrapc(bx, g.n[0]), byw = wrapc(by, g.n[1]), bzw = wrapc(bz, g.n[2]);
                 bool sunk = false;
                 int dex = 0, dey = 0, dez = 0;
                 if (MODE == MODE_PUSH) {
                     float Ex, Ey, Ez, Bx, By, Bz;
-                    gather_direct(A, bxw, byw, bzw, px, py, pz, Ex, Ey, Ez, Bx, By, Bz);
+                    if ((unsigned)blx < (unsigned)g.T[0] && (unsigned)bly < (unsigned)g.T[1] &&
+                        (unsigned)blz < (unsigned)g.T[2])
+                        gather_lds(sF4, sF2, blx * fsx + bly * fsy + blz, fsx, fsy, px, py, pz, Ex, Ey, Ez, Bx, By, Bz);
+                    else
+                        gather_direct(A, bxw, byw, bzw, px, py, pz, Ex, Ey, Ez, Bx, By, Bz);
                     boris_move(g, qom, subk, Ex, Ey, Ez, Bx, By, Bz, vx, vy, vz, px, py, pz);
                     sunk = !finite3(px, py, pz);
                     if (!sunk) {
@@ -736,7 +783,7 @@ __global__ __launch_bounds__(NT) void k_particles(PushArgs A) {
                         atomicOr(A.err, 1);
                         ok = false;
                     } else {
-                        store_particle(S, oidx, sunk ? NaNf : px, py, pz, vx, vy, vz);
+                        store_particle(rout, cs, oidx, sunk ? NaNf : px, py, pz, vx, vy, vz);
                         if (TRACK) S.id_out[oidx] = S.id_in[inq];
                     }
                 }
@@ -745,8 +792,8 @@ __global__ __launch_bounds__(NT) void k_particles(PushArgs A) {
                 } else if (MODE != MODE_REBIN && ok) {
                     float wt[8];
                     tri_weights(px - (float)dex, py - (float)dey, pz - (float)dez, wt);
-                    deposit_direct(A, sJ, x0, y0, z0, bx - x0 + dex, by - y0 + dey, bz - z0 + dez, wt, qs * vx,
-                                   qs * vy, qs * vz);
+                    deposit_direct(A, sJ, x0, y0, z0, blx + dex, bly + dey, blz + dez, wt, qs * vx, qs * vy,
+                                   qs * vz);
                     if (MODE == MODE_PUSH) {
                         count_add(g, s_cnt, S.cnt_acc, x0, y0, z0, bx + dex, by + dey, bz + dez);
                         if ((dex | dey | dez) == 0) count_add(g, s_stay, S.stay_acc, x0, y0, z0, bx, by, bz);
@@ -1314,10 +1361,8 @@ PushArgs push_args(ipic_ctx* ctx) {
     for (int s = 0; s < A.nsp; ++s) {
         Species& S = ctx->sp[s];
         SpArgs& a = A.sp[s];
-        for (int k = 0; k < 6; ++k) {
-            a.in[k] = S.X[S.cur] + (size_t)k * S.cap;
-            a.out[k] = S.X[1 - S.cur] + (size_t)k * S.cap;
-        }
+        a.in = S.X[S.cur];
+        a.out = S.X[1 - S.cur];
         a.id_in = S.ids[S.cur];
         a.id_out = S.ids[1 - S.cur];
         a.cap = S.cap;
@@ -1376,7 +1421,8 @@ ScanArgs scan_args(ipic_ctx* ctx) {
 int launch_particles(ipic_ctx* ctx, int mode) {
     PushArgs A = push_args(ctx);
     dim3 grid(ctx->g.ntiles), block(NT);
-    const size_t lds = (size_t)particle_layout(ctx->g.NJ, ctx->g.TC).total;
+    const size_t lds =
+        (size_t)particle_layout(ctx->g.NJ, ctx->g.TC, mode == MODE_PUSH ? field_nodes(ctx->g) : 0).total;
     if (ctx->track) {
         if (mode == MODE_PUSH) hipLaunchKernelGGL((k_particles<MODE_PUSH, true>), grid, block, lds, ctx->stream, A);
         else if (mode == MODE_DEPOSIT) hipLaunchKernelGGL((k_particles<MODE_DEPOSIT, true>), grid, block, lds, ctx->stream, A);
@@ -1472,6 +1518,9 @@ int alloc_species_buffers(ipic_ctx* ctx, int s, long long cap) {
             S.ids[b] = nullptr;
         }
         const long long c = std::max<long long>(cap, 1);
+        // the six arrays of a species are addressed through one buffer resource (< 4 GiB)
+        if (c * 24 >= (1LL << 32))
+            return fail(ctx, IPIC_E_RANGE, "a species holds at most 178956970 particles");
         for (int b = 0; b < 2; ++b) {
             if (int rc = dalloc(ctx, &S.X[b], (size_t)6 * c)) return rc;
             if (ctx->track)

@@ -54,10 +54,10 @@ def load_traffic(cells: int, ppc: int):
         return None
 
 
-def make_sim(G, cells, ppc, seed, flags=0):
+def make_sim(G, cells, ppc, seed, flags=0, tile=(0, 0, 0)):
     half = ppc // 2
     q = 1.0 / (half * FOUR_PI)
-    sim = G.Simulation(cells, dt=0.05, c=1.0, species=((-q, -1.0), (q, 1.0 / 64)), flags=flags)
+    sim = G.Simulation(cells, dt=0.05, c=1.0, species=((-q, -1.0), (q, 1.0 / 64)), flags=flags, tile=tile)
     sim.init_particles(0, half, (0.05,) * 3, seed=seed)
     sim.init_particles(1, ppc - half, (0.006,) * 3, seed=seed)
     rng = np.random.default_rng(seed)
@@ -102,6 +102,7 @@ def copy_peak_gbs(G):
     try:
         import torch
         if not torch.cuda.is_available():
+            log("copy peak unavailable: torch sees no GPU")
             return None
         n = 1 << 28  # 1 GiB of fp32
         a = torch.empty(n, dtype=torch.float32, device="cuda")
@@ -130,6 +131,8 @@ def main():
     ap.add_argument("--cells", type=int, default=128)
     ap.add_argument("--ppc", type=int, default=32)
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--tile", type=int, nargs=3, default=(0, 0, 0),
+                    help="cells per tile along x y z (0: the library's default)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-copy-peak", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -155,13 +158,14 @@ def main():
         print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "launches": 10}))
         return
 
+    copy_peak = None if args.no_copy_peak else copy_peak_gbs(G)
     info = G.device_info()
     log("device:", info)
     cells, ppc = args.cells, args.ppc
     npart = cells ** 3 * ppc
 
     log(f"init {cells}^3 x {ppc} ppc = {npart} particles")
-    sim = make_sim(G, cells, ppc, args.seed)
+    sim = make_sim(G, cells, ppc, args.seed, tile=tuple(args.tile))
     if args.warmup:
         sim.step(args.warmup)
     log(f"warmup {args.warmup} steps done")
@@ -173,7 +177,7 @@ def main():
     sim.close()
 
     # the dominant kernel's launch duration, measured with HIP events on the library's stream
-    simt = make_sim(G, cells, ppc, args.seed, flags=G.FLAG_PHASE_TIMING)
+    simt = make_sim(G, cells, ppc, args.seed, flags=G.FLAG_PHASE_TIMING, tile=tuple(args.tile))
     simt.step(max(2, args.warmup))
     nt = min(args.steps, 20)
     stt = simt.step(nt)
@@ -187,7 +191,6 @@ def main():
     log(f"push kernel {push_ms:.3f} ms/launch -> {achieved:.1f} GB/s algorithmic "
         f"({achieved / HBM_PEAK_GBS * 100:.1f}% of {HBM_PEAK_GBS:.0f} GB/s); field {field_ms:.3f} ms, scan {scan_ms:.3f} ms")
 
-    copy_peak = None if args.no_copy_peak else copy_peak_gbs(G)
     cpu = None if args.no_cpu_baseline else cpu_baseline(cells, ppc)
     if cpu:
         log("cpu baseline:", cpu)
