@@ -16,7 +16,6 @@ LIB_PATH = os.path.join(_HERE, "_lib", "libipic_gpu.so")
 MAX_SPECIES = 8
 
 FLAG_TRACK_IDS = 1
-FLAG_NO_GRAPH = 2
 FLAG_PHASE_TIMING = 4
 
 
@@ -53,6 +52,13 @@ class StepStats(C.Structure):
         ("ms_push", C.c_double),
         ("ms_scan", C.c_double),
         ("ms_deposit", C.c_double),
+        ("operator_steps", C.c_int64),
+        ("fallback_steps", C.c_int64),
+        ("audits", C.c_int64),
+        ("max_audit_error", C.c_double),
+        ("max_audit_error_increment", C.c_double),
+        ("max_divb_residual", C.c_double),
+        ("max_energy_residual", C.c_double),
     ]
 
     def as_dict(self):
@@ -76,12 +82,34 @@ class Conservation(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class FieldOperatorDesc(C.Structure):
+    _fields_ = [
+        ("params", C.c_void_p),
+        ("in_E", C.c_float), ("in_B", C.c_float), ("in_J", C.c_float),
+        ("out_E", C.c_float), ("out_P", C.c_float),
+        ("substeps", C.c_int32),
+        ("audit_every", C.c_int32),
+        ("divb_tol", C.c_double),
+        ("energy_tol", C.c_double),
+        ("audit_tol", C.c_double),
+    ]
+
+
+class FieldState(C.Structure):
+    _fields_ = [
+        ("E", C.c_void_p), ("Bc", C.c_void_p), ("J", C.c_void_p),
+        ("E0", C.c_double * 3), ("B0", C.c_double * 3),
+        ("uniform_background", C.c_int32),
+    ]
+
+
 # every symbol declared in include/ipic_gpu.h
 ABI_SYMBOLS = (
     "ipic_gpu_create", "ipic_gpu_destroy", "ipic_gpu_last_error", "ipic_gpu_upload_particles",
     "ipic_gpu_init_particles", "ipic_gpu_upload_fields", "ipic_gpu_step", "ipic_gpu_num_particles",
     "ipic_gpu_cell_counts", "ipic_gpu_download_particles", "ipic_gpu_download_fields",
-    "ipic_gpu_conservation", "ipic_gpu_device_info",
+    "ipic_gpu_conservation", "ipic_gpu_device_info", "ipic_gpu_field_step", "ipic_gpu_particle_step",
+    "ipic_gpu_set_field_operator", "ipic_gpu_field_state",
 )
 
 _lib = None
@@ -114,6 +142,10 @@ def load_library(path: str = LIB_PATH):
     L.ipic_gpu_download_fields.argtypes = [P, P, P, P, P]
     L.ipic_gpu_conservation.argtypes = [P, C.POINTER(Conservation)]
     L.ipic_gpu_device_info.argtypes = [C.c_char_p, I32, C.POINTER(C.c_int32), C.POINTER(C.c_int64)]
+    L.ipic_gpu_field_step.argtypes = [P, I32, C.POINTER(StepStats)]
+    L.ipic_gpu_particle_step.argtypes = [P, C.POINTER(StepStats)]
+    L.ipic_gpu_set_field_operator.argtypes = [P, C.POINTER(FieldOperatorDesc)]
+    L.ipic_gpu_field_state.argtypes = [P, C.POINTER(FieldState)]
     for name in ABI_SYMBOLS:
         if name != "ipic_gpu_last_error":
             getattr(L, name).restype = C.c_int
@@ -228,6 +260,41 @@ class Simulation:
         st = StepStats()
         self._check(self._L.ipic_gpu_step(self._h, int(nsteps), C.byref(st)))
         return st.as_dict()
+
+    def field_step(self, use_operator=False, stats=None):
+        """One field phase (leapfrog substeps, or the field operator); returns the stats."""
+        st = stats if stats is not None else StepStats()
+        self._check(self._L.ipic_gpu_field_step(self._h, 1 if use_operator else 0, C.byref(st)))
+        return st
+
+    def particle_step(self, stats=None):
+        st = stats if stats is not None else StepStats()
+        self._check(self._L.ipic_gpu_particle_step(self._h, C.byref(st)))
+        return st
+
+    def set_field_operator(self, params=None, scales=None, substeps=None, audit_every=0, divb_tol=1e-4,
+                           energy_tol=1e-3, audit_tol=1e-3):
+        """Installs the learned field step (params: FP32 vector, scales: dict with in_E, in_B,
+        in_J, out_E, out_P); params=None switches it off."""
+        if params is None:
+            self._check(self._L.ipic_gpu_set_field_operator(self._h, None))
+            return
+        p = np.ascontiguousarray(np.asarray(params, dtype=np.float32).reshape(-1))
+        d = FieldOperatorDesc()
+        d.params = p.ctypes.data_as(C.c_void_p)
+        for k in ("in_E", "in_B", "in_J", "out_E", "out_P"):
+            setattr(d, k, float(scales[k]))
+        d.substeps = int(self.desc.field_substeps if substeps is None else substeps)
+        d.audit_every = int(audit_every)
+        d.divb_tol, d.energy_tol, d.audit_tol = float(divb_tol), float(energy_tol), float(audit_tol)
+        self._check(self._L.ipic_gpu_set_field_operator(self._h, C.byref(d)))
+
+    def field_state(self):
+        """Device pointers of dE, dBc, J (float4 x n^3) and the uniform background."""
+        fs = FieldState()
+        self._check(self._L.ipic_gpu_field_state(self._h, C.byref(fs)))
+        return {"E": fs.E, "Bc": fs.Bc, "J": fs.J, "E0": tuple(fs.E0), "B0": tuple(fs.B0),
+                "uniform_background": bool(fs.uniform_background)}
 
     # ---- state out
     def num_particles(self, species=-1):

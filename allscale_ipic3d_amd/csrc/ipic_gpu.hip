@@ -26,6 +26,7 @@
 #include <vector>
 
 #include "ipic_gpu.h"
+#include "ipic_nop.h"
 
 namespace {
 
@@ -869,9 +870,52 @@ __device__ __forceinline__ void xyz_of(const Geo& g, int i, int& x, int& y, int&
     z = i % g.n[2];
 }
 
-// E pass of field.h:359-363: E += dt (c^2 curl- Bc - J).  With from_partials the current
-// density of each node is first summed in FP64 over every tile partial covering it (the
-// cross-tile moment sum) plus the overflow accumulator, and stored to J.
+// The cross-tile moment sum: J of node (x,y,z) summed in FP64 over every tile partial covering
+// it plus the overflow accumulator (cleared here), rounded once to FP32.
+__device__ __forceinline__ float4 sum_partials(const FieldArgs& F, int i, int x, int y, int z) {
+    const Geo& g = F.g;
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    const int* px = F.jpairs + (0 * g.nmax + x) * (1 + JPMAX);
+    const int* py = F.jpairs + (1 * g.nmax + y) * (1 + JPMAX);
+    const int* pz = F.jpairs + (2 * g.nmax + z) * (1 + JPMAX);
+    for (int a = 0; a < px[0]; ++a)
+        for (int b = 0; b < py[0]; ++b)
+            for (int cc = 0; cc < pz[0]; ++cc) {
+                const int ta = px[1 + a] >> 8, la = px[1 + a] & 255;
+                const int tb = py[1 + b] >> 8, lb = py[1 + b] & 255;
+                const int tc = pz[1 + cc] >> 8, lc = pz[1 + cc] & 255;
+                const int tile = (ta * g.NTl[1] + tb) * g.NTl[2] + tc;
+                const int l = (la * g.nj[1] + lb) * g.nj[2] + lc;
+                const float* base = F.jpart + (long long)tile * 3 * g.NJ;
+                sx += (double)base[l];
+                sy += (double)base[g.NJ + l];
+                sz += (double)base[2 * g.NJ + l];
+            }
+    if (*F.jflag) {
+        sx += (double)F.jover[3 * i + 0];
+        sy += (double)F.jover[3 * i + 1];
+        sz += (double)F.jover[3 * i + 2];
+        F.jover[3 * i + 0] = 0.0f;
+        F.jover[3 * i + 1] = 0.0f;
+        F.jover[3 * i + 2] = 0.0f;
+    }
+    return make_float4((float)sx, (float)sy, (float)sz, 0.0f);
+}
+
+// J from the tile partials, without a field update (ahead of the field operator).
+__global__ __launch_bounds__(256) void k_jsum(FieldArgs F) {
+    const Geo& g = F.g;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= g.ncell) return;
+    int x, y, z;
+    xyz_of(g, i, x, y, z);
+    F.J[i] = sum_partials(F, i, x, y, z);
+}
+
+__global__ void k_clear_flag(int* f) { *f = 0; }
+
+// E pass of field.h:359-363: E += dt (c^2 curl- Bc - J).  With from_partials J is first
+// formed from the tile partials (sum_partials) and stored.
 __global__ __launch_bounds__(256) void k_field_E(FieldArgs F, int from_partials) {
     const Geo& g = F.g;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -880,33 +924,9 @@ __global__ __launch_bounds__(256) void k_field_E(FieldArgs F, int from_partials)
     xyz_of(g, i, x, y, z);
     float Jx, Jy, Jz;
     if (from_partials) {
-        double sx = 0.0, sy = 0.0, sz = 0.0;
-        const int* px = F.jpairs + (0 * g.nmax + x) * (1 + JPMAX);
-        const int* py = F.jpairs + (1 * g.nmax + y) * (1 + JPMAX);
-        const int* pz = F.jpairs + (2 * g.nmax + z) * (1 + JPMAX);
-        for (int a = 0; a < px[0]; ++a)
-            for (int b = 0; b < py[0]; ++b)
-                for (int cc = 0; cc < pz[0]; ++cc) {
-                    const int ta = px[1 + a] >> 8, la = px[1 + a] & 255;
-                    const int tb = py[1 + b] >> 8, lb = py[1 + b] & 255;
-                    const int tc = pz[1 + cc] >> 8, lc = pz[1 + cc] & 255;
-                    const int tile = (ta * g.NTl[1] + tb) * g.NTl[2] + tc;
-                    const int l = (la * g.nj[1] + lb) * g.nj[2] + lc;
-                    const float* base = F.jpart + (long long)tile * 3 * g.NJ;
-                    sx += (double)base[l];
-                    sy += (double)base[g.NJ + l];
-                    sz += (double)base[2 * g.NJ + l];
-                }
-        if (*F.jflag) {
-            sx += (double)F.jover[3 * i + 0];
-            sy += (double)F.jover[3 * i + 1];
-            sz += (double)F.jover[3 * i + 2];
-            F.jover[3 * i + 0] = 0.0f;
-            F.jover[3 * i + 1] = 0.0f;
-            F.jover[3 * i + 2] = 0.0f;
-        }
-        Jx = (float)sx; Jy = (float)sy; Jz = (float)sz;
-        F.J[i] = make_float4(Jx, Jy, Jz, 0.0f);
+        const float4 j4 = sum_partials(F, i, x, y, z);
+        Jx = j4.x; Jy = j4.y; Jz = j4.z;
+        F.J[i] = j4;
     } else {
         const float4 j4 = F.J[i];
         Jx = j4.x; Jy = j4.y; Jz = j4.z;
@@ -1229,6 +1249,73 @@ __global__ __launch_bounds__(256) void k_reduce_fields(FieldArgs F, double* out)
     }
 }
 
+// Checks of a field-operator step, FP64 per-block partials: sum over nodes/centres of
+// |E0 + dE|^2 + c^2 |dBc|^2 before and after, of (E0 + dE) . J before and after, and
+// max |div+ (dBc' - dBc)| (the divergence the update adds).
+__global__ __launch_bounds__(256) void k_op_check(FieldArgs F, const float4* Eold, const float4* Bold, double* out) {
+    const Geo& g = F.g;
+    __shared__ double s[4];
+    double wo = 0, wn = 0, ejo = 0, ejn = 0, md = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < g.ncell; i += gridDim.x * blockDim.x) {
+        int x, y, z;
+        xyz_of(g, i, x, y, z);
+        const float4 eo = Eold[i], en = F.E[i], bo = Bold[i], bn = F.Bc[i], j = F.J[i];
+        const double ox = (double)g.E0[0] + eo.x, oy = (double)g.E0[1] + eo.y, oz = (double)g.E0[2] + eo.z;
+        const double nx = (double)g.E0[0] + en.x, ny = (double)g.E0[1] + en.y, nz = (double)g.E0[2] + en.z;
+        wo += ox * ox + oy * oy + oz * oz + F.c2 * ((double)bo.x * bo.x + (double)bo.y * bo.y + (double)bo.z * bo.z);
+        wn += nx * nx + ny * ny + nz * nz + F.c2 * ((double)bn.x * bn.x + (double)bn.y * bn.y + (double)bn.z * bn.z);
+        ejo += ox * j.x + oy * j.y + oz * j.z;
+        ejn += nx * j.x + ny * j.y + nz * j.z;
+        const int ipx = nid_of(g, wrapc(x + 1, g.n[0]), y, z), ipy = nid_of(g, x, wrapc(y + 1, g.n[1]), z),
+                  ipz = nid_of(g, x, y, wrapc(z + 1, g.n[2]));
+        const double d0x = (double)bn.x - bo.x, d0y = (double)bn.y - bo.y, d0z = (double)bn.z - bo.z;
+        const double dpx = (double)F.Bc[ipx].x - Bold[ipx].x;
+        const double dpy = (double)F.Bc[ipy].y - Bold[ipy].y;
+        const double dpz = (double)F.Bc[ipz].z - Bold[ipz].z;
+        const double dv = (dpx - d0x) * g.inv_w[0] + (dpy - d0y) * g.inv_w[1] + (dpz - d0z) * g.inv_w[2];
+        md = fmax(md, fabs(dv));
+    }
+    wo = block_sum_d(wo, s);
+    wn = block_sum_d(wn, s);
+    ejo = block_sum_d(ejo, s);
+    ejn = block_sum_d(ejn, s);
+    md = block_max_d(md, s);
+    if (threadIdx.x == 0) {
+        double* o = out + 5 * blockIdx.x;
+        o[0] = wo; o[1] = wn; o[2] = ejo; o[3] = ejn; o[4] = md;
+    }
+}
+
+// Audit sums: |Fa - Fb|^2, |Fb - Fc|^2 and |Fb + F0|^2 with |F|^2 = |E|^2 + c^2 |Bc|^2 (a: operator
+// result, b: leapfrog result, c: state before the field phase, F0: uniform background).
+__global__ __launch_bounds__(256) void k_audit(Geo g, float c2, const float4* Ea, const float4* Ba,
+                                               const float4* Eb, const float4* Bb, const float4* Ec,
+                                               const float4* Bc, double* out) {
+    __shared__ double s[4];
+    const int ncell = g.ncell;
+    double num = 0, den = 0, tot = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < ncell; i += gridDim.x * blockDim.x) {
+        const float4 ea = Ea[i], ba = Ba[i], eb = Eb[i], bb = Bb[i], ec = Ec[i], bc = Bc[i];
+        const double a0 = (double)ea.x - eb.x, a1 = (double)ea.y - eb.y, a2 = (double)ea.z - eb.z;
+        const double a3 = (double)ba.x - bb.x, a4 = (double)ba.y - bb.y, a5 = (double)ba.z - bb.z;
+        const double b0 = (double)eb.x - ec.x, b1 = (double)eb.y - ec.y, b2 = (double)eb.z - ec.z;
+        const double b3 = (double)bb.x - bc.x, b4 = (double)bb.y - bc.y, b5 = (double)bb.z - bc.z;
+        num += a0 * a0 + a1 * a1 + a2 * a2 + c2 * (a3 * a3 + a4 * a4 + a5 * a5);
+        den += b0 * b0 + b1 * b1 + b2 * b2 + c2 * (b3 * b3 + b4 * b4 + b5 * b5);
+        const double t0 = (double)g.E0[0] + eb.x, t1 = (double)g.E0[1] + eb.y, t2 = (double)g.E0[2] + eb.z;
+        const double t3 = (double)g.B0[0] + bb.x, t4 = (double)g.B0[1] + bb.y, t5 = (double)g.B0[2] + bb.z;
+        tot += t0 * t0 + t1 * t1 + t2 * t2 + c2 * (t3 * t3 + t4 * t4 + t5 * t5);
+    }
+    num = block_sum_d(num, s);
+    den = block_sum_d(den, s);
+    tot = block_sum_d(tot, s);
+    if (threadIdx.x == 0) {
+        out[3 * blockIdx.x] = num;
+        out[3 * blockIdx.x + 1] = den;
+        out[3 * blockIdx.x + 2] = tot;
+    }
+}
+
 __global__ void k_fill_f4(float4* p, long long n, float4 v) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
@@ -1292,6 +1379,21 @@ struct ipic_ctx {
     std::vector<void*> allocs;
     std::string err;
     hipEvent_t ev[8] = {};
+    // field operator
+    bool op_on = false;
+    ipic_field_operator op{};      // settings (params pointer not kept)
+    ipic_nop_scales op_scales{};
+    long long op_count = 0;        // operator applications since it was set
+    float* op_params = nullptr;
+    uint16_t* op_wf = nullptr;
+    unsigned char* op_ws = nullptr;
+    uint16_t* op_u = nullptr;
+    float* op_out = nullptr;
+    float4* op_psi = nullptr;
+    float4* Eb = nullptr;          // state before the field phase
+    float4* Bb = nullptr;
+    float4* Eo = nullptr;          // operator result during an audit
+    float4* Bo = nullptr;
 };
 
 namespace {
@@ -1448,15 +1550,178 @@ int launch_scan(ipic_ctx* ctx) {
     return 0;
 }
 
-int launch_field(ipic_ctx* ctx) {
+int launch_leapfrog(ipic_ctx* ctx) {
     FieldArgs F = field_args(ctx);
     const int nb = (ctx->g.ncell + 255) / 256;
     for (int s = 0; s < ctx->d.field_substeps; ++s) {
         hipLaunchKernelGGL(k_field_E, dim3(nb), dim3(256), 0, ctx->stream, F, s == 0 ? 1 : 0);
         hipLaunchKernelGGL(k_field_B, dim3(nb), dim3(256), 0, ctx->stream, F);
     }
+    HIPCK(ctx, hipGetLastError());
+    return 0;
+}
+
+int launch_c2n(ipic_ctx* ctx) {
+    FieldArgs F = field_args(ctx);
+    const int nb = (ctx->g.ncell + 255) / 256;
     hipLaunchKernelGGL(k_c2n, dim3(nb), dim3(256), 0, ctx->stream, F);
     HIPCK(ctx, hipGetLastError());
+    return 0;
+}
+
+// J formed from the tile partials (idempotent: the overflow accumulator is consumed)
+int launch_jsum(ipic_ctx* ctx) {
+    FieldArgs F = field_args(ctx);
+    const int nb = (ctx->g.ncell + 255) / 256;
+    hipLaunchKernelGGL(k_jsum, dim3(nb), dim3(256), 0, ctx->stream, F);
+    hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(1), 0, ctx->stream, ctx->jflag);
+    HIPCK(ctx, hipGetLastError());
+    return 0;
+}
+
+int copy_f4(ipic_ctx* ctx, float4* dst, const float4* src) {
+    HIPCK(ctx, hipMemcpyAsync(dst, src, (size_t)ctx->g.ncell * sizeof(float4), hipMemcpyDeviceToDevice, ctx->stream));
+    return 0;
+}
+
+int nop_fail(ipic_ctx* ctx, int rc) { return fail(ctx, rc, std::string("field operator: ") + ipic_nop_last_error()); }
+
+// One operator application with its checks, fallback and audit (the field phase without C2N).
+int operator_phase(ipic_ctx* ctx, ipic_step_stats* st) {
+    const Geo& g = ctx->g;
+    if (ctx->SE || ctx->SB || ctx->Ebg || ctx->Bcbg)
+        return fail(ctx, IPIC_E_STATE, "the field operator needs a uniform background field");
+    if (int rc = launch_jsum(ctx)) return rc;
+    if (int rc = copy_f4(ctx, ctx->Eb, ctx->E)) return rc;
+    if (int rc = copy_f4(ctx, ctx->Bb, ctx->Bc)) return rc;
+    const int32_t n3[3] = {g.n[0], g.n[1], g.n[2]};
+    const float iw[3] = {g.inv_w[0], g.inv_w[1], g.inv_w[2]};
+    if (int rc = ipic_nop_pack(n3, (const float*)ctx->E, (const float*)ctx->Bc, (const float*)ctx->J, &ctx->op_scales,
+                               ctx->op_u, ctx->stream))
+        return nop_fail(ctx, rc);
+    if (int rc = ipic_nop_forward(n3, ctx->op_u, ctx->op_wf, ctx->op_params, ctx->op_ws, 0, ctx->op_out, ctx->stream))
+        return nop_fail(ctx, rc);
+    if (int rc = ipic_nop_apply(n3, ctx->op_out, &ctx->op_scales, iw, (float*)ctx->E, (float*)ctx->Bc,
+                                (float*)ctx->op_psi, ctx->stream))
+        return nop_fail(ctx, rc);
+    // checks
+    FieldArgs F = field_args(ctx);
+    hipLaunchKernelGGL(k_op_check, dim3(ctx->red_blocks), dim3(256), 0, ctx->stream, F, ctx->Eb, ctx->Bb, ctx->red);
+    HIPCK(ctx, hipGetLastError());
+    std::vector<double> h((size_t)ctx->red_blocks * 5);
+    HIPCK(ctx, hipMemcpyAsync(h.data(), ctx->red, h.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCK(ctx, hipStreamSynchronize(ctx->stream));
+    double wo = 0, wn = 0, ejo = 0, ejn = 0, md = 0;
+    for (int b = 0; b < ctx->red_blocks; ++b) {
+        wo += h[5 * b];
+        wn += h[5 * b + 1];
+        ejo += h[5 * b + 2];
+        ejn += h[5 * b + 3];
+        md = std::max(md, h[5 * b + 4]);
+    }
+    const double vol = ctx->d.cell_width[0] * ctx->d.cell_width[1] * ctx->d.cell_width[2];
+    const double W0 = 0.5 * wo * vol, W1 = 0.5 * wn * vol;
+    const double eres = std::fabs(W1 - W0 + ctx->d.dt * 0.5 * (ejo + ejn) * vol) / std::max(std::max(W0, W1), 1e-300);
+    const bool ok = md <= ctx->op.divb_tol && eres <= ctx->op.energy_tol;  // false for NaN
+    if (st) {
+        st->max_divb_residual = std::max(st->max_divb_residual, std::isfinite(md) ? md : HUGE_VAL);
+        st->max_energy_residual = std::max(st->max_energy_residual, std::isfinite(eres) ? eres : HUGE_VAL);
+    }
+    ++ctx->op_count;
+    if (!ok) {
+        if (int rc = copy_f4(ctx, ctx->E, ctx->Eb)) return rc;
+        if (int rc = copy_f4(ctx, ctx->Bc, ctx->Bb)) return rc;
+        if (int rc = launch_leapfrog(ctx)) return rc;
+        if (st) st->fallback_steps++;
+        return 0;
+    }
+    if (ctx->op.audit_every > 0 && ctx->op_count % ctx->op.audit_every == 0) {
+        if (int rc = copy_f4(ctx, ctx->Eo, ctx->E)) return rc;
+        if (int rc = copy_f4(ctx, ctx->Bo, ctx->Bc)) return rc;
+        if (int rc = copy_f4(ctx, ctx->E, ctx->Eb)) return rc;
+        if (int rc = copy_f4(ctx, ctx->Bc, ctx->Bb)) return rc;
+        if (int rc = launch_leapfrog(ctx)) return rc;
+        hipLaunchKernelGGL(k_audit, dim3(ctx->red_blocks), dim3(256), 0, ctx->stream, g, F.c2, ctx->Eo, ctx->Bo,
+                           ctx->E, ctx->Bc, ctx->Eb, ctx->Bb, ctx->red);
+        HIPCK(ctx, hipGetLastError());
+        HIPCK(ctx, hipMemcpyAsync(h.data(), ctx->red, (size_t)ctx->red_blocks * 3 * sizeof(double),
+                                  hipMemcpyDeviceToHost, ctx->stream));
+        HIPCK(ctx, hipStreamSynchronize(ctx->stream));
+        double num = 0, den = 0, tot = 0;
+        for (int b = 0; b < ctx->red_blocks; ++b) {
+            num += h[3 * b];
+            den += h[3 * b + 1];
+            tot += h[3 * b + 2];
+        }
+        auto ratio = [](double a, double b) { return b > 0 ? std::sqrt(a / b) : (a > 0 ? HUGE_VAL : 0.0); };
+        const double err = ratio(num, tot), err_inc = ratio(num, den);
+        if (st) {
+            st->audits++;
+            st->max_audit_error = std::max(st->max_audit_error, std::isfinite(err) ? err : HUGE_VAL);
+            st->max_audit_error_increment =
+                std::max(st->max_audit_error_increment, std::isfinite(err_inc) ? err_inc : HUGE_VAL);
+        }
+        if (err <= ctx->op.audit_tol) {
+            if (int rc = copy_f4(ctx, ctx->E, ctx->Eo)) return rc;
+            if (int rc = copy_f4(ctx, ctx->Bc, ctx->Bo)) return rc;
+        } else {
+            if (st) st->fallback_steps++;
+            return 0;
+        }
+    }
+    if (st) st->operator_steps++;
+    return 0;
+}
+
+double ev_ms(ipic_ctx* ctx, int a, int b) {
+    float ms = 0.0f;
+    if (hipEventSynchronize(ctx->ev[b]) != hipSuccess) return 0.0;
+    if (hipEventElapsedTime(&ms, ctx->ev[a], ctx->ev[b]) != hipSuccess) return 0.0;
+    return ms;
+}
+
+// Field phase of a step: initial deposit when needed, leapfrog or operator, interpC2N.
+int field_phase(ipic_ctx* ctx, bool use_op, ipic_step_stats* st) {
+    const bool timing = (ctx->d.flags & IPIC_FLAG_PHASE_TIMING) != 0;
+    if (!ctx->j_valid) {
+        if (timing) HIPCK(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
+        if (int rc = launch_particles(ctx, MODE_DEPOSIT)) return rc;
+        if (timing) {
+            HIPCK(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
+            if (st) st->ms_deposit += ev_ms(ctx, 0, 1);
+        }
+        ctx->j_valid = true;
+    }
+    if (timing) HIPCK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
+    if (use_op && ctx->op_on) {
+        if (int rc = operator_phase(ctx, st)) return rc;
+    } else {
+        if (int rc = launch_leapfrog(ctx)) return rc;
+    }
+    if (int rc = launch_c2n(ctx)) return rc;
+    if (timing) {
+        HIPCK(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
+        if (st) st->ms_field += ev_ms(ctx, 2, 3);
+    }
+    return 0;
+}
+
+// Particle phase: fused push kernel, then the cell-count scans.
+int particle_phase(ipic_ctx* ctx, ipic_step_stats* st) {
+    const bool timing = (ctx->d.flags & IPIC_FLAG_PHASE_TIMING) != 0;
+    if (timing) HIPCK(ctx, hipEventRecord(ctx->ev[4], ctx->stream));
+    if (int rc = launch_particles(ctx, MODE_PUSH)) return rc;
+    if (timing) HIPCK(ctx, hipEventRecord(ctx->ev[5], ctx->stream));
+    if (int rc = launch_scan(ctx)) return rc;
+    if (timing) {
+        HIPCK(ctx, hipEventRecord(ctx->ev[6], ctx->stream));
+        if (st) {
+            st->ms_push += ev_ms(ctx, 4, 5);
+            st->ms_scan += ev_ms(ctx, 5, 6);
+        }
+    }
+    ctx->canonical = false;
+    ctx->j_valid = true;  // the push deposited the J of the new state
     return 0;
 }
 
@@ -1949,44 +2214,14 @@ int ipic_gpu_upload_fields(ipic_ctx* ctx, const double* E, const double* Bc) {
 int ipic_gpu_step(ipic_ctx* ctx, int64_t nsteps, ipic_step_stats* stats) {
     if (int rc = check_ctx(ctx)) return rc;
     if (nsteps < 0) return fail(ctx, IPIC_E_ARG, "negative step count");
-    const bool timing = (ctx->d.flags & IPIC_FLAG_PHASE_TIMING) != 0;
     ipic_step_stats st{};
     HIPCK(ctx, hipMemsetAsync(ctx->removed, 0, sizeof(unsigned long long), ctx->stream));
     HIPCK(ctx, hipStreamSynchronize(ctx->stream));
-    double ms_field = 0, ms_push = 0, ms_scan = 0, ms_dep = 0;
     auto t0 = std::chrono::steady_clock::now();
     auto t1 = t0;
     for (int64_t s = 0; s < nsteps; ++s) {
-        if (!ctx->j_valid) {
-            if (timing) HIPCK(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
-            if (int rc = launch_particles(ctx, MODE_DEPOSIT)) return rc;
-            if (timing) {
-                HIPCK(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
-                HIPCK(ctx, hipEventSynchronize(ctx->ev[1]));
-                float ms = 0;
-                HIPCK(ctx, hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
-                ms_dep += ms;
-            }
-            ctx->j_valid = true;
-        }
-        if (timing) HIPCK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
-        if (int rc = launch_field(ctx)) return rc;
-        if (timing) HIPCK(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
-        if (int rc = launch_particles(ctx, MODE_PUSH)) return rc;
-        if (timing) HIPCK(ctx, hipEventRecord(ctx->ev[4], ctx->stream));
-        if (int rc = launch_scan(ctx)) return rc;
-        if (timing) {
-            HIPCK(ctx, hipEventRecord(ctx->ev[5], ctx->stream));
-            HIPCK(ctx, hipEventSynchronize(ctx->ev[5]));
-            float a = 0, b = 0, c = 0;
-            HIPCK(ctx, hipEventElapsedTime(&a, ctx->ev[2], ctx->ev[3]));
-            HIPCK(ctx, hipEventElapsedTime(&b, ctx->ev[3], ctx->ev[4]));
-            HIPCK(ctx, hipEventElapsedTime(&c, ctx->ev[4], ctx->ev[5]));
-            ms_field += a;
-            ms_push += b;
-            ms_scan += c;
-        }
-        ctx->canonical = false;
+        if (int rc = field_phase(ctx, true, &st)) return rc;
+        if (int rc = particle_phase(ctx, &st)) return rc;
         if (s == 0) {
             HIPCK(ctx, hipStreamSynchronize(ctx->stream));
             t1 = std::chrono::steady_clock::now();
@@ -2001,10 +2236,6 @@ int ipic_gpu_step(ipic_ctx* ctx, int64_t nsteps, ipic_step_stats* stats) {
     st.remaining_steps_s = std::chrono::duration<double>(t2 - t1).count();
     st.steps = nsteps;
     st.removed = (int64_t)removed;
-    st.ms_field = ms_field;
-    st.ms_push = ms_push;
-    st.ms_scan = ms_scan;
-    st.ms_deposit = ms_dep;
     long long tot = 0;
     for (int s = 0; s < ctx->d.nspecies; ++s) {
         long long t = 0;
@@ -2013,6 +2244,107 @@ int ipic_gpu_step(ipic_ctx* ctx, int64_t nsteps, ipic_step_stats* stats) {
     }
     st.particles = tot;
     if (stats) *stats = st;
+    return 0;
+}
+
+int ipic_gpu_field_step(ipic_ctx* ctx, int32_t use_operator, ipic_step_stats* stats) {
+    if (int rc = check_ctx(ctx)) return rc;
+    if (use_operator && !ctx->op_on) return fail(ctx, IPIC_E_STATE, "no field operator set");
+    ipic_step_stats st{};
+    if (int rc = field_phase(ctx, use_operator != 0, &st)) return rc;
+    HIPCK(ctx, hipStreamSynchronize(ctx->stream));
+    if (stats) {
+        stats->ms_field += st.ms_field;
+        stats->ms_deposit += st.ms_deposit;
+        stats->operator_steps += st.operator_steps;
+        stats->fallback_steps += st.fallback_steps;
+        stats->audits += st.audits;
+        stats->max_audit_error = std::max(stats->max_audit_error, st.max_audit_error);
+        stats->max_audit_error_increment = std::max(stats->max_audit_error_increment, st.max_audit_error_increment);
+        stats->max_divb_residual = std::max(stats->max_divb_residual, st.max_divb_residual);
+        stats->max_energy_residual = std::max(stats->max_energy_residual, st.max_energy_residual);
+    }
+    return 0;
+}
+
+int ipic_gpu_particle_step(ipic_ctx* ctx, ipic_step_stats* stats) {
+    if (int rc = check_ctx(ctx)) return rc;
+    ipic_step_stats st{};
+    HIPCK(ctx, hipMemsetAsync(ctx->removed, 0, sizeof(unsigned long long), ctx->stream));
+    if (int rc = particle_phase(ctx, &st)) return rc;
+    HIPCK(ctx, hipStreamSynchronize(ctx->stream));
+    if (int rc = check_device_flag(ctx)) return rc;
+    if (stats) {
+        unsigned long long removed = 0;
+        HIPCK(ctx, hipMemcpy(&removed, ctx->removed, sizeof(removed), hipMemcpyDeviceToHost));
+        stats->removed += (int64_t)removed;
+        stats->ms_push += st.ms_push;
+        stats->ms_scan += st.ms_scan;
+        stats->steps += 1;
+    }
+    return 0;
+}
+
+int ipic_gpu_set_field_operator(ipic_ctx* ctx, const ipic_field_operator* op) {
+    if (int rc = check_ctx(ctx)) return rc;
+    if (!op) {
+        ctx->op_on = false;
+        return 0;
+    }
+    if (!op->params) return fail(ctx, IPIC_E_ARG, "null operator parameters");
+    if (op->substeps != ctx->d.field_substeps)
+        return fail(ctx, IPIC_E_ARG, "operator substeps differ from the context's field_substeps");
+    if (!(op->in_E > 0) || !(op->in_B > 0) || !(op->in_J > 0) || !std::isfinite(op->out_E) || !std::isfinite(op->out_P))
+        return fail(ctx, IPIC_E_ARG, "operator scales must be positive and finite");
+    if (op->audit_every < 0) return fail(ctx, IPIC_E_ARG, "negative audit interval");
+    const Geo& g = ctx->g;
+    const int32_t n3[3] = {g.n[0], g.n[1], g.n[2]};
+    const long long wsb = ipic_nop_workspace_bytes(n3, 0);
+    if (wsb < 0) return nop_fail(ctx, IPIC_E_ARG);
+    const long long np = ipic_nop_param_count();
+    const size_t nc = (size_t)g.ncell;
+    if (!ctx->op_params) {
+        if (int rc = dalloc(ctx, &ctx->op_params, (size_t)np)) return rc;
+        if (int rc = dalloc(ctx, &ctx->op_wf, (size_t)np)) return rc;
+        if (int rc = dalloc(ctx, &ctx->op_ws, (size_t)wsb)) return rc;
+        if (int rc = dalloc(ctx, &ctx->op_u, nc * 16)) return rc;
+        if (int rc = dalloc(ctx, &ctx->op_out, nc * 16)) return rc;
+        if (int rc = dalloc(ctx, &ctx->op_psi, nc)) return rc;
+        if (int rc = dalloc(ctx, &ctx->Eb, nc)) return rc;
+        if (int rc = dalloc(ctx, &ctx->Bb, nc)) return rc;
+        if (int rc = dalloc(ctx, &ctx->Eo, nc)) return rc;
+        if (int rc = dalloc(ctx, &ctx->Bo, nc)) return rc;
+    }
+    for (long long i = 0; i < np; ++i)
+        if (!std::isfinite(op->params[i])) return fail(ctx, IPIC_E_ARG, "non-finite operator parameter");
+    HIPCK(ctx, hipMemcpyAsync(ctx->op_params, op->params, (size_t)np * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+    if (int rc = ipic_nop_prepare(ctx->op_params, ctx->op_wf, nullptr, ctx->stream)) return nop_fail(ctx, rc);
+    HIPCK(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->op = *op;
+    ctx->op.params = nullptr;
+    ctx->op_scales = ipic_nop_scales{op->in_E, op->in_B, op->in_J, op->out_E, op->out_P};
+    ctx->op_count = 0;
+    ctx->op_on = true;
+    return 0;
+}
+
+int ipic_gpu_field_state(ipic_ctx* ctx, ipic_field_state* out) {
+    if (int rc = check_ctx(ctx)) return rc;
+    if (!out) return fail(ctx, IPIC_E_ARG, "null output");
+    if (!ctx->j_valid) {
+        if (int rc = launch_particles(ctx, MODE_DEPOSIT)) return rc;
+        ctx->j_valid = true;
+    }
+    if (int rc = launch_jsum(ctx)) return rc;
+    HIPCK(ctx, hipStreamSynchronize(ctx->stream));
+    out->E = (float*)ctx->E;
+    out->Bc = (float*)ctx->Bc;
+    out->J = (float*)ctx->J;
+    for (int k = 0; k < 3; ++k) {
+        out->E0[k] = ctx->g.E0[k];
+        out->B0[k] = ctx->g.B0[k];
+    }
+    out->uniform_background = (ctx->Ebg || ctx->Bcbg || ctx->SE || ctx->SB) ? 0 : 1;
     return 0;
 }
 

@@ -38,8 +38,7 @@ extern "C" {
 
 /* flags */
 #define IPIC_FLAG_TRACK_IDS 1u    /* carry a uint32 id per particle (input order) */
-#define IPIC_FLAG_NO_GRAPH 2u     /* launch kernels one by one instead of a HIP graph */
-#define IPIC_FLAG_PHASE_TIMING 4u /* record per-phase HIP events (implies NO_GRAPH) */
+#define IPIC_FLAG_PHASE_TIMING 4u /* record per-phase HIP events (synchronises after each step) */
 
 /* Particle species: every particle of a species shares Particle::q and Particle::qom
  * (particle.h:15-16). */
@@ -74,10 +73,20 @@ typedef struct ipic_step_stats {
     int64_t particles;            /* live particles after the call, all species */
     int64_t removed;              /* particles removed by the sink during the call */
     /* filled when IPIC_FLAG_PHASE_TIMING is set: summed GPU milliseconds per phase */
-    double ms_field;              /* leapfrog E/B passes, interpC2N, cross-tile J sum */
+    double ms_field;              /* field phase: leapfrog substeps or the field operator with its
+                                     checks, interpC2N, cross-tile J sum */
     double ms_push;               /* fused gather + Boris + rebin + deposit kernel */
     double ms_scan;               /* cell-count prefix sums */
     double ms_deposit;            /* initial deposit of a fresh particle state */
+    /* field operator (ipic_gpu_set_field_operator) */
+    int64_t operator_steps;       /* field phases whose result came from the operator */
+    int64_t fallback_steps;       /* operator results replaced by leapfrog (failed check or audit) */
+    int64_t audits;               /* operator steps compared against leapfrog */
+    double max_audit_error;       /* largest audited |F_op - F_lf| / |F_lf| of the total field
+                                     (background included), |F|^2 = sum |E|^2 + c^2 |Bc|^2 */
+    double max_audit_error_increment; /* the same error relative to |F_lf - F_before| */
+    double max_divb_residual;     /* largest max |div+ (Bc_after - Bc_before)| of an operator step */
+    double max_energy_residual;   /* largest field-energy residual of an operator step */
 } ipic_step_stats;
 
 /* FP64 conservation sums, reduced on the GPU (cell.h:1315-1355, field.h:443-459). */
@@ -143,6 +152,49 @@ int ipic_gpu_download_fields(ipic_ctx* ctx, double* E, double* Bc, double* Bn, d
 
 /* FP64 conservation sums of the current state. */
 int ipic_gpu_conservation(ipic_ctx* ctx, ipic_conservation* out);
+
+/* The two halves of a step (ipic_gpu_step runs field then particles, nsteps times).
+ * Field phase: the J of the last deposit summed over tiles, then `field_substeps` leapfrog
+ * substeps (field.h:356-372) with J held fixed -- or, with use_operator = 1 and an operator
+ * set, the operator with its checks -- then interpC2N for the gather.  Particle phase: gather
+ * + Boris + rebin + deposit and the cell-count scans.  stats (nullable) accumulate. */
+int ipic_gpu_field_step(ipic_ctx* ctx, int32_t use_operator, ipic_step_stats* stats);
+int ipic_gpu_particle_step(ipic_ctx* ctx, ipic_step_stats* stats);
+
+/* Learned field step (no reference counterpart; operator and parameter layout in
+ * ipic_nop.h).  With an operator set, every field phase of ipic_gpu_step runs it in place of
+ * the leapfrog substeps and falls back to leapfrog when max |div+ (Bc' - Bc)| > divb_tol or
+ * when the field-energy residual |W' - W + dt (E.J + E'.J)/2 vol| / max(W, W') > energy_tol,
+ * where W = 1/2 sum (|E0 + dE|^2 + c^2 |dBc|^2) vol is the energy of the field without its
+ * uniform magnetic background.  Every audit_every-th operator step is also advanced by
+ * leapfrog from the same state; the operator's result is kept when its error relative to the
+ * leapfrog field (|F_op - F_lf| / |F_lf|, total field) is at most audit_tol, else the leapfrog
+ * result is.  Needs a uniform
+ * background field (upload_fields with spatially constant E, Bc), nx % 4 == 0, ny % 4 == 0,
+ * nz % 16 == 0 and substeps == field_substeps.  op == NULL switches the operator off. */
+typedef struct ipic_field_operator {
+    const float* params;          /* host FP32 parameter vector (ipic_nop_param_count() floats) */
+    float in_E, in_B, in_J;       /* input scales */
+    float out_E, out_P;           /* output scales */
+    int32_t substeps;             /* leapfrog substeps one application stands for */
+    int32_t audit_every;          /* 0: no audits */
+    double divb_tol;
+    double energy_tol;
+    double audit_tol;
+} ipic_field_operator;
+int ipic_gpu_set_field_operator(ipic_ctx* ctx, const ipic_field_operator* op);
+
+/* Device view of the field state for a trainer (valid until the next call on ctx): float4
+ * arrays of n^3 entries, (x, y, z, unused).  J is formed from the last deposit first. */
+typedef struct ipic_field_state {
+    float* E;                     /* E - E0 on nodes */
+    float* Bc;                    /* Bc - B0 on centres */
+    float* J;                     /* current density on nodes */
+    double E0[3];                 /* uniform background */
+    double B0[3];
+    int32_t uniform_background;   /* 1: the background is E0, B0 only */
+} ipic_field_state;
+int ipic_gpu_field_state(ipic_ctx* ctx, ipic_field_state* out);
 
 /* Device properties used by the tests and benchmark. */
 int ipic_gpu_device_info(char* name, int32_t name_len, int32_t* cu_count, int64_t* hbm_bytes);
