@@ -50,6 +50,30 @@ def build(verbose: bool = False) -> list[str]:
     return out
 
 
+CPP_TESTS = {
+    # C++ mirror of the reference's step API, driven by tests/test_cpp_mirror.py
+    os.path.join(ROOT, "tests", "cpp", "ipic3d_mirror_test"): os.path.join(ROOT, "tests", "cpp", "ipic3d_mirror_test.cpp"),
+}
+
+
+def build_cpp(verbose: bool = False) -> list[str]:
+    """Host programs over the C ABI (g++, linked against the in-tree libipic_gpu.so)."""
+    out = []
+    for dst, src in CPP_TESTS.items():
+        rpath = os.path.relpath(LIBDIR, os.path.dirname(dst))
+        cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-I", os.path.join(ROOT, "include"), "-o", dst + ".tmp", src,
+               "-L", LIBDIR, "-lipic_gpu", f"-Wl,-rpath,$ORIGIN/{rpath}"]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            sys.stderr.write(r.stdout + r.stderr)
+            raise RuntimeError(f"g++ failed for {src}")
+        os.replace(dst + ".tmp", dst)
+        out.append(dst)
+    return out
+
+
 if __name__ == "__main__":
-    for p in build(verbose=True):
+    for p in build(verbose=True) + build_cpp(verbose=True):
         print("built", p)

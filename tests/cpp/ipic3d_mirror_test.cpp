@@ -1,0 +1,91 @@
+// Drives the C++ mirror (include/ipic3d_gpu/ipic3d.hpp) the way the reference's simulation
+// tests drive theirs: build a Universe, place particles, set fields, simulateSteps.
+//
+//   ipic3d_mirror_test <input.bin> <output.bin>   run the case of input.bin on the GPU
+//   ipic3d_mirror_test --expect-no-device          check the error path without a GPU
+//
+// input.bin (little-endian): int64 n, int64 steps, int64 nparticles, double dt, then per
+// particle 8 doubles (position, velocity, q, qom), then n^3 x 3 doubles E and n^3 x 3 Bc.
+// output.bin: int64 particle count, n^3 int64 cell counts, double kinetic energy, double
+// firstStep, double remainingSteps, int64 1 if the operator-policy guard threw.
+#include <cstdio>
+#include <fstream>
+#include <iostream>
+#include <set>
+
+#include "ipic3d_gpu/ipic3d.hpp"
+
+using namespace ipic3d_gpu;
+
+int main(int argc, char** argv) {
+    if (argc == 2 && std::string(argv[1]) == "--expect-no-device") {
+        UniverseProperties p;
+        p.size = {4, 4, 4};
+        try {
+            Universe u(p, {{-1.0, -1.0}});
+            std::printf("{\"created\": true}\n");
+            return 1;
+        } catch (const Error& e) {
+            std::printf("{\"error\": %d}\n", e.code());
+            return e.code() == IPIC_E_HIP ? 0 : 1;
+        }
+    }
+    if (argc != 3) {
+        std::fprintf(stderr, "usage: %s input.bin output.bin | --expect-no-device\n", argv[0]);
+        return 2;
+    }
+    std::ifstream in(argv[1], std::ios::binary);
+    int64_t n = 0, steps = 0, np = 0;
+    double dt = 0;
+    in.read((char*)&n, 8).read((char*)&steps, 8).read((char*)&np, 8).read((char*)&dt, 8);
+    std::vector<Particle> ps((size_t)np);
+    std::set<std::pair<double, double>> kinds;
+    for (auto& p : ps) {
+        double b[8];
+        in.read((char*)b, sizeof(b));
+        p.position = {b[0], b[1], b[2]};
+        p.velocity = {b[3], b[4], b[5]};
+        p.q = b[6];
+        p.qom = b[7];
+        kinds.insert({p.q, p.qom});
+    }
+    const size_t nc = (size_t)(n * n * n);
+    std::vector<Vector3> E(nc), Bc(nc);
+    in.read((char*)E[0].data(), nc * 24);
+    in.read((char*)Bc[0].data(), nc * 24);
+    if (!in) {
+        std::fprintf(stderr, "short input\n");
+        return 2;
+    }
+    try {
+        UniverseProperties props;
+        props.size = {n, n, n};
+        props.dt = dt;
+        Universe u(props, std::vector<std::pair<double, double>>(kinds.begin(), kinds.end()));
+        u.setParticles(ps);
+        u.setFields(E, Bc);
+        int64_t guard = 0;
+        try {
+            simulateSteps<detail::gpu_particle_to_field_projector, detail::gpu_field_operator_solver>(1, u);
+        } catch (const Error& e) {
+            guard = e.code() == IPIC_E_STATE ? 1 : 0;
+        }
+        const DurationMeasurement d = simulateSteps(steps, u);
+        const int64_t count = u.countParticlesInDomain();
+        const std::vector<int64_t> cells = u.getParticleCountsPerCell();
+        const double ke = u.conservation().kinetic_energy;
+        std::ofstream out(argv[2], std::ios::binary);
+        out.write((const char*)&count, 8);
+        out.write((const char*)cells.data(), (std::streamsize)(cells.size() * 8));
+        out.write((const char*)&ke, 8);
+        out.write((const char*)&d.firstStep, 8);
+        out.write((const char*)&d.remainingSteps, 8);
+        out.write((const char*)&guard, 8);
+        std::printf("{\"particles\": %lld, \"kinetic_energy\": %.17g, \"guard\": %lld}\n", (long long)count, ke,
+                    (long long)guard);
+    } catch (const Error& e) {
+        std::fprintf(stderr, "error %d: %s\n", e.code(), e.what());
+        return 1;
+    }
+    return 0;
+}
