@@ -17,6 +17,8 @@ MAX_SPECIES = 8
 
 FLAG_TRACK_IDS = 1
 FLAG_PHASE_TIMING = 4
+FLAG_REFLECT = 8
+FLAG_UNIFORM_BACKGROUND = 16
 
 
 class SpeciesDesc(C.Structure):

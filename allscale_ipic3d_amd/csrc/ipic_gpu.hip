@@ -30,10 +30,26 @@
 
 namespace {
 
-constexpr int NT = 256;          // threads per block of the particle kernels
+// tuning knobs of the particle kernel (compile-time)
+#ifndef IPIC_PUSH_THREADS
+#define IPIC_PUSH_THREADS 256    // threads per block of the particle kernel
+#endif
+#ifndef IPIC_KRUN
+#define IPIC_KRUN 5              // max particles per run (one lane, held in registers)
+#endif
+#ifndef IPIC_JHALO
+#define IPIC_JHALO 0             // node layers of the tile J accumulator beyond the tile (0 or 1);
+                                 // deposits beyond it go to the global overflow accumulator
+#endif
+#ifndef IPIC_PUSH_WAVES_PER_EU
+#define IPIC_PUSH_WAVES_PER_EU 3 // register budget of the particle kernel (0: compiler's choice)
+#endif
+
+constexpr int NT = 256;          // threads per block of the scan kernels
 constexpr int WS = 64;           // wavefront size
-constexpr int NW = NT / WS;      // waves per block
-constexpr int KRUN = 7;          // max particles per run (one lane, held in registers)
+constexpr int NTP = IPIC_PUSH_THREADS;
+constexpr int NW = NTP / WS;     // waves per block of the particle kernel
+constexpr int KRUN = IPIC_KRUN;
 constexpr int WCH = WS * KRUN;   // max particles per wave chunk
 constexpr int K0CAP = 32;        // leaver buffer entries per wave chunk
 constexpr int TCMAX = 512;       // max cells per tile
@@ -50,7 +66,8 @@ struct Geo {
     int ntiles;
     int ncell;
     int NJ;
-    int nj[3];
+    int nj[3];            // J accumulator nodes per axis: T + 1 + 2 jh
+    int jh;               // its halo (node layers beyond the tile on each side)
     int nmax;
     int s0;               // n[1] * n[2], row stride of x
     float w[3];
@@ -60,6 +77,7 @@ struct Geo {
     float inv_vol;
     float pi_c;
     int sink;
+    int reflect;          // 1: reflect at the global faces (cell.h:1103-1121) instead of wrapping
     float sink_c[3];
     float sink_r2;
     float E0[3];          // uniform part of the background field (added in the gather)
@@ -202,7 +220,7 @@ __device__ __forceinline__ void deposit_direct(const PushArgs& A, long long* sJ,
 #pragma unroll
             for (int k = 0; k < 2; ++k) {
                 const float ww = wt[(i * 2 + j) * 2 + k];
-                const int nx = elx + 1 + i, ny = ely + 1 + j, nz = elz + 1 + k;
+                const int nx = elx + g.jh + i, ny = ely + g.jh + j, nz = elz + g.jh + k;
                 if (nx >= 0 && nx < g.nj[0] && ny >= 0 && ny < g.nj[1] && nz >= 0 && nz < g.nj[2]) {
                     const int l = (nx * g.nj[1] + ny) * g.nj[2] + nz;
                     fix_add(&sJ[l], jx * ww, g.fix_pre);
@@ -344,6 +362,21 @@ __device__ __forceinline__ void boris_move(const Geo& g, float qom, float subk, 
     }
 }
 
+// Reflection at the global faces (cell.h:1103-1121) for a particle of cell (cx,cy,cz) at offsets
+// (px,py,pz): past the low face of cell 0 (offset < 0) it moves half a cell back (+0.5), past
+// the high face of cell n-1 (offset > 1) half a cell back (-0.5); either inverts the whole
+// velocity once.
+__device__ __forceinline__ void reflect_faces(const Geo& g, int cx, int cy, int cz, float& px, float& py, float& pz,
+                                              float& vx, float& vy, float& vz) {
+    bool inv = false;
+    if (cx == 0 && px < 0.0f) { px += 0.5f; inv = true; } else if (cx == g.n[0] - 1 && px > 1.0f) { px -= 0.5f; inv = true; }
+    if (cy == 0 && py < 0.0f) { py += 0.5f; inv = true; } else if (cy == g.n[1] - 1 && py > 1.0f) { py -= 0.5f; inv = true; }
+    if (cz == 0 && pz < 0.0f) { pz += 0.5f; inv = true; } else if (cz == g.n[2] - 1 && pz > 1.0f) { pz -= 0.5f; inv = true; }
+    if (inv) {
+        vx = -vx; vy = -vy; vz = -vz;
+    }
+}
+
 // E and B (gather arrays plus the uniform background, divided by the cell volume) at offsets
 // (x,y,z) of the cell with wrapped coordinates (cx,cy,cz), read straight from the node arrays.
 __device__ __forceinline__ void gather_direct(const PushArgs& A, int cx, int cy, int cz, float x, float y,
@@ -462,8 +495,14 @@ __host__ __device__ inline PLayout particle_layout(int NJ, int TC, int NF) {
 
 __host__ __device__ inline int field_nodes(const Geo& g) { return (g.T[0] + 1) * (g.T[1] + 1) * (g.T[2] + 1); }
 
+#if IPIC_PUSH_WAVES_PER_EU > 0
+#define IPIC_PUSH_ATTR __attribute__((amdgpu_waves_per_eu(IPIC_PUSH_WAVES_PER_EU)))
+#else
+#define IPIC_PUSH_ATTR
+#endif
+
 template <int MODE, bool TRACK>
-__global__ __launch_bounds__(NT) void k_particles(PushArgs A) {
+__global__ __launch_bounds__(NTP) IPIC_PUSH_ATTR void k_particles(PushArgs A) {
     const Geo& g = A.g;
     extern __shared__ __align__(16) unsigned char smem[];
     const PLayout L = particle_layout(g.NJ, g.TC, MODE == MODE_PUSH ? field_nodes(g) : 0);
@@ -489,11 +528,11 @@ __global__ __launch_bounds__(NT) void k_particles(PushArgs A) {
     const int fsy = g.T[2] + 1, fsx = (g.T[1] + 1) * fsy;  // node strides of the LDS field
 
     if (MODE != MODE_REBIN)
-        for (int i = tid; i < 3 * g.NJ; i += NT) sJ[i] = 0;
+        for (int i = tid; i < 3 * g.NJ; i += NTP) sJ[i] = 0;
     if (MODE == MODE_PUSH) {
         // the tile's (T+1)^3 nodes of E and B, background added, divided by the cell volume
         const int nf = (g.T[0] + 1) * fsx;
-        for (int i = tid; i < nf; i += NT) {
+        for (int i = tid; i < nf; i += NTP) {
             const int ix = i / fsx, r = i - ix * fsx, iy = r / fsy, iz = r - iy * fsy;
             const int nd = wrap1(x0 + ix, g.n[0]) * g.s0 + wrap1(y0 + iy, g.n[1]) * g.n[2] + wrap1(z0 + iz, g.n[2]);
             const float3 e = ld3(A.E + nd), b = ld3(A.Bn + nd);
@@ -509,8 +548,8 @@ __global__ __launch_bounds__(NT) void k_particles(PushArgs A) {
         const Rsrc rout = make_rsrc(S.out, S.cap * 24);
         const unsigned cs = (unsigned)S.cap * 4u;  // byte stride between the six arrays
         __syncthreads();
-        for (int i = tid; i <= g.TC; i += NT) s_seg[i] = S.seg[cbase + i];
-        for (int i = tid; i < g.TC; i += NT) {
+        for (int i = tid; i <= g.TC; i += NTP) s_seg[i] = S.seg[cbase + i];
+        for (int i = tid; i < g.TC; i += NTP) {
             s_cnt[i] = 0;
             s_stay[i] = 0;
         }
@@ -639,6 +678,7 @@ __global__ __launch_bounds__(NT) void k_particles(PushArgs A) {
                             float Ex, Ey, Ez, Bx, By, Bz;
                             gather_lds(sF4, sF2, nb, fsx, fsy, px, py, pz, Ex, Ey, Ez, Bx, By, Bz);
                             boris_move(g, qom, subk, Ex, Ey, Ez, Bx, By, Bz, vx, vy, vz, px, py, pz);
+                            if (g.reflect) reflect_faces(g, ax, ay, az, px, py, pz, vx, vy, vz);
                             sunk = !finite3(px, py, pz);
                             if (g.sink && !sunk) {
                                 const int dex = dcell(px), dey = dcell(py), dez = dcell(pz);
@@ -711,7 +751,7 @@ __global__ __launch_bounds__(NT) void k_particles(PushArgs A) {
                         for (int j = 0; j < 2; ++j)
 #pragma unroll
                             for (int kk = 0; kk < 2; ++kk) {
-                                const int l = ((lx + 1 + i) * g.nj[1] + (ly + 1 + j)) * g.nj[2] + (lz + 1 + kk);
+                                const int l = ((lx + g.jh + i) * g.nj[1] + (ly + g.jh + j)) * g.nj[2] + (lz + g.jh + kk);
                                 fix_add(&sJ[d * g.NJ + l], N[(i * 2 + j) * 2 + kk], g.fix_pre);
                             }
                 }
@@ -765,6 +805,7 @@ __global__ __launch_bounds__(NT) void k_particles(PushArgs A) {
                     else
                         gather_direct(A, bxw, byw, bzw, px, py, pz, Ex, Ey, Ez, Bx, By, Bz);
                     boris_move(g, qom, subk, Ex, Ey, Ez, Bx, By, Bz, vx, vy, vz, px, py, pz);
+                    if (g.reflect) reflect_faces(g, bxw, byw, bzw, px, py, pz, vx, vy, vz);
                     sunk = !finite3(px, py, pz);
                     if (!sunk) {
                         dex = dcell(px); dey = dcell(py); dez = dcell(pz);
@@ -822,7 +863,7 @@ __global__ __launch_bounds__(NT) void k_particles(PushArgs A) {
         }
         __syncthreads();
         if (MODE == MODE_PUSH) {
-            for (int i = tid; i < g.TC; i += NT) {
+            for (int i = tid; i < g.TC; i += NTP) {
                 if (s_cnt[i]) atomicAdd(&S.cnt_acc[cbase + i], s_cnt[i]);
                 if (s_stay[i]) atomicAdd(&S.stay_acc[cbase + i], s_stay[i]);
             }
@@ -831,7 +872,7 @@ __global__ __launch_bounds__(NT) void k_particles(PushArgs A) {
     __syncthreads();
     if (MODE != MODE_REBIN) {
         float* dst = A.jpart + (long long)t * 3 * g.NJ;
-        for (int i = tid; i < 3 * g.NJ; i += NT) dst[i] = (float)((double)sJ[i] * g.fix_inv);
+        for (int i = tid; i < 3 * g.NJ; i += NTP) dst[i] = (float)((double)sJ[i] * g.fix_inv);
     }
 }
 
@@ -913,6 +954,13 @@ __global__ __launch_bounds__(256) void k_jsum(FieldArgs F) {
 }
 
 __global__ void k_clear_flag(int* f) { *f = 0; }
+
+// Drops overflow deposits that were never summed (a deposit pass replaces the partials).
+__global__ void k_discard_overflow(const int* flag, float* jover, long long n) {
+    if (!*flag) return;
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) jover[i] = 0.0f;
+}
 
 // E pass of field.h:359-363: E += dt (c^2 curl- Bc - J).  With from_partials J is first
 // formed from the tile partials (sum_partials) and stored.
@@ -1374,6 +1422,7 @@ struct ipic_ctx {
     int red_blocks = 1024;
     Species sp[IPIC_MAX_SPECIES];
     bool j_valid = false;
+    bool j_summed = true;          // J holds the sum of the current tile partials (and overflow)
     bool canonical = true;
     bool track = false;
     std::vector<void*> allocs;
@@ -1443,7 +1492,7 @@ int build_jpairs(const Geo& g, std::vector<int>& tab) {
         for (int x = 0; x < g.n[d]; ++x) {
             int* e = &tab[((size_t)d * g.nmax + x) * (1 + JPMAX)];
             for (int l = 0; l < g.nj[d]; ++l) {
-                int s = (x + 1 - l) % g.n[d];
+                int s = (x + g.jh - l) % g.n[d];
                 if (s < 0) s += g.n[d];
                 if (s % g.T[d] == 0) {
                     if (e[0] >= JPMAX) return -1;
@@ -1521,8 +1570,14 @@ ScanArgs scan_args(ipic_ctx* ctx) {
 }
 
 int launch_particles(ipic_ctx* ctx, int mode) {
+    if (mode != MODE_REBIN && !ctx->j_summed) {
+        const long long n = 3LL * ctx->g.ncell;
+        hipLaunchKernelGGL(k_discard_overflow, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, ctx->jflag,
+                           ctx->jover, n);
+        hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(1), 0, ctx->stream, ctx->jflag);
+    }
     PushArgs A = push_args(ctx);
-    dim3 grid(ctx->g.ntiles), block(NT);
+    dim3 grid(ctx->g.ntiles), block(NTP);
     const size_t lds =
         (size_t)particle_layout(ctx->g.NJ, ctx->g.TC, mode == MODE_PUSH ? field_nodes(ctx->g) : 0).total;
     if (ctx->track) {
@@ -1535,6 +1590,7 @@ int launch_particles(ipic_ctx* ctx, int mode) {
         else hipLaunchKernelGGL((k_particles<MODE_REBIN, false>), grid, block, lds, ctx->stream, A);
     }
     HIPCK(ctx, hipGetLastError());
+    if (mode != MODE_REBIN) ctx->j_summed = false;
     if (mode != MODE_DEPOSIT)
         for (int s = 0; s < ctx->d.nspecies; ++s) ctx->sp[s].cur = 1 - ctx->sp[s].cur;
     return 0;
@@ -1554,10 +1610,13 @@ int launch_leapfrog(ipic_ctx* ctx) {
     FieldArgs F = field_args(ctx);
     const int nb = (ctx->g.ncell + 255) / 256;
     for (int s = 0; s < ctx->d.field_substeps; ++s) {
-        hipLaunchKernelGGL(k_field_E, dim3(nb), dim3(256), 0, ctx->stream, F, s == 0 ? 1 : 0);
+        // the first E pass forms J from the partials unless that was done already
+        const int from_partials = (s == 0 && !ctx->j_summed) ? 1 : 0;
+        hipLaunchKernelGGL(k_field_E, dim3(nb), dim3(256), 0, ctx->stream, F, from_partials);
         hipLaunchKernelGGL(k_field_B, dim3(nb), dim3(256), 0, ctx->stream, F);
     }
     HIPCK(ctx, hipGetLastError());
+    ctx->j_summed = true;
     return 0;
 }
 
@@ -1569,13 +1628,16 @@ int launch_c2n(ipic_ctx* ctx) {
     return 0;
 }
 
-// J formed from the tile partials (idempotent: the overflow accumulator is consumed)
+// J formed from the tile partials and the overflow accumulator (which it consumes), once per
+// deposit
 int launch_jsum(ipic_ctx* ctx) {
+    if (ctx->j_summed) return 0;
     FieldArgs F = field_args(ctx);
     const int nb = (ctx->g.ncell + 255) / 256;
     hipLaunchKernelGGL(k_jsum, dim3(nb), dim3(256), 0, ctx->stream, F);
     hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(1), 0, ctx->stream, ctx->jflag);
     HIPCK(ctx, hipGetLastError());
+    ctx->j_summed = true;
     return 0;
 }
 
@@ -1908,7 +1970,8 @@ int ipic_gpu_create(const ipic_desc* desc, ipic_ctx** out) {
         }
         g.T[k] = T;
         g.NTl[k] = d.n[k] / T;
-        g.nj[k] = T + 3;
+        g.jh = IPIC_JHALO;
+        g.nj[k] = T + 1 + 2 * g.jh;
         g.w[k] = (float)d.cell_width[k];
         g.inv_w[k] = (float)(1.0 / d.cell_width[k]);
         g.origin[k] = (float)d.origin[k];
@@ -1938,6 +2001,7 @@ int ipic_gpu_create(const ipic_desc* desc, ipic_ctx** out) {
         g.fix_inv = std::ldexp(1.0, -bits);
     }
     g.sink = d.sink ? 1 : 0;
+    g.reflect = (d.flags & IPIC_FLAG_REFLECT) ? 1 : 0;
     g.sink_r2 = (float)(d.sink_radius * d.sink_radius);
 
     std::vector<int> jp;
@@ -2143,6 +2207,28 @@ int ipic_gpu_upload_fields(ipic_ctx* ctx, const double* E, const double* Bc) {
             nuE |= rE[3 * i + k] != 0.0;
             nuB |= rB[3 * i + k] != 0.0;
         }
+    if (ctx->d.flags & IPIC_FLAG_UNIFORM_BACKGROUND) {
+        // only E0, B0 stay static; the remainders start the evolving FP32 state
+        for (int i = 0; i < nc; ++i)
+            for (int k = 0; k < 3; ++k) {
+                ctx->bgE[3 * i + k] = (double)ctx->g.E0[k];
+                ctx->bgBc[3 * i + k] = (double)ctx->g.B0[k];
+            }
+        std::vector<float4> he(nc), hb(nc);
+        for (int i = 0; i < nc; ++i) {
+            he[i] = make_float4((float)rE[3 * i], (float)rE[3 * i + 1], (float)rE[3 * i + 2], 0.0f);
+            hb[i] = make_float4((float)rB[3 * i], (float)rB[3 * i + 1], (float)rB[3 * i + 2], 0.0f);
+        }
+        for (float4** a : {&ctx->Ebg, &ctx->Eg, &ctx->Bcbg, &ctx->Bnbg, &ctx->SE, &ctx->SB}) {
+            dfree(ctx, *a);
+            *a = nullptr;
+        }
+        HIPCK(ctx, hipMemcpyAsync(ctx->E, he.data(), nc * sizeof(float4), hipMemcpyHostToDevice, ctx->stream));
+        HIPCK(ctx, hipMemcpyAsync(ctx->Bc, hb.data(), nc * sizeof(float4), hipMemcpyHostToDevice, ctx->stream));
+        if (int rc = launch_c2n(ctx)) return rc;
+        HIPCK(ctx, hipStreamSynchronize(ctx->stream));
+        return 0;
+    }
     // static sources of the background: SE = c^2 curl- Bc_bg (field.h:359-363),
     // SB = curl+ E_bg (field.h:367-371), FP64
     std::vector<double> sE(n3), sB(n3), rBn(n3);

@@ -273,8 +273,9 @@ def config2_simulation(n, ppc, seed, field_substeps=8, dt=0.05, sigma_E=1e-3, B0
     vth 0.006), ppc/2 each per cell, B = B0, E ~ N(0, sigma_E^2) per node component."""
     half = ppc // 2
     q = 1.0 / (half * 4.0 * np.pi)
+    from . import FLAG_UNIFORM_BACKGROUND
     sim = Simulation(n, dt=dt, c=1.0, species=((-q, -1.0), (q, 1.0 / 64)), field_substeps=field_substeps,
-                     flags=flags)
+                     flags=flags | FLAG_UNIFORM_BACKGROUND)
     sim.init_particles(0, half, (0.05,) * 3, seed=seed)
     sim.init_particles(1, ppc - half, (0.006,) * 3, seed=seed)
     rng = np.random.default_rng(seed)

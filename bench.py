@@ -123,6 +123,54 @@ def copy_peak_gbs(G):
         return None
 
 
+OPERATOR_FILE = os.path.join(ROOT, "allscale_ipic3d_amd", "data", "operator_64.npz")
+
+
+def surrogate_leg(G, cells, ppc, phases=10, seed=21):
+    """Field phase of the learned operator against 8 leapfrog substeps on the same state.
+
+    Times one field phase each way with HIP events on the library's stream (the operator's
+    phase includes the J sum, pack, network, head, the div B / energy checks with their host
+    read-back, and interpC2N), then audits every operator application against leapfrog for
+    the error |F_op - F_lf| / |F_lf| (total field) and the increment-relative error."""
+    from allscale_ipic3d_amd import field_operator as FO
+    trained = os.path.exists(OPERATOR_FILE)
+    if trained:
+        from allscale_ipic3d_amd.train_operator import load
+        params, scales, meta = load(OPERATOR_FILE)
+    else:
+        params, scales, meta = FO.init_params(0), {"in_E": 1e-3, "in_B": 1e-3, "in_J": 1e-3, "out_E": 1e-5,
+                                                   "out_P": 5e-5}, {}
+    sim = FO.config2_simulation(cells, ppc, seed, field_substeps=8, flags=G.FLAG_PHASE_TIMING)
+    sim.step(2)
+    lf = G.StepStats()
+    for _ in range(phases):
+        sim.field_step(use_operator=False, stats=lf)
+        sim.particle_step()
+    sim.set_field_operator(params, scales, divb_tol=1e-4, energy_tol=1e-3)
+    op = G.StepStats()
+    for _ in range(phases):
+        sim.field_step(use_operator=True, stats=op)
+        sim.particle_step()
+    sim.set_field_operator(params, scales, divb_tol=1e30, energy_tol=1e30, audit_every=1, audit_tol=1e30)
+    au = G.StepStats()
+    for _ in range(phases):
+        sim.field_step(use_operator=True, stats=au)
+        sim.particle_step()
+    sim.close()
+    t_lf, t_op = lf.ms_field / phases, op.ms_field / phases
+    flops = FO.FLOP_PER_CELL * cells ** 3
+    return {
+        "workload": f"{cells}^3 cells x {ppc} ppc two-species field phase, 8 leapfrog substeps (J fixed) vs one "
+                    "operator application", "trained": trained, "training": meta,
+        "t_leapfrog_ms": t_lf, "t_operator_ms": t_op, "speedup": t_lf / t_op if t_op > 0 else None,
+        "operator_steps": int(op.operator_steps), "fallback_steps": int(op.fallback_steps),
+        "max_divb_residual": op.max_divb_residual, "max_energy_residual": op.max_energy_residual,
+        "rel_error_total": au.max_audit_error, "rel_error_increment": au.max_audit_error_increment,
+        "flop_per_inference": flops, "operator_phase_tflops": flops / (t_op * 1e-3) / 1e12 if t_op > 0 else None,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -135,6 +183,8 @@ def main():
                     help="cells per tile along x y z (0: the library's default)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-copy-peak", action="store_true")
+    ap.add_argument("--no-surrogate", action="store_true", help="skip the learned field step leg")
+    ap.add_argument("--surrogate-cells", type=int, default=64)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy 10 times (rocprof counter calibration)")
     args = ap.parse_args()
@@ -191,6 +241,10 @@ def main():
     log(f"push kernel {push_ms:.3f} ms/launch -> {achieved:.1f} GB/s algorithmic "
         f"({achieved / HBM_PEAK_GBS * 100:.1f}% of {HBM_PEAK_GBS:.0f} GB/s); field {field_ms:.3f} ms, scan {scan_ms:.3f} ms")
 
+    sur = None
+    if not args.no_surrogate:
+        sur = surrogate_leg(G, args.surrogate_cells, 64)
+        log("surrogate:", {k: v for k, v in sur.items() if k != "training"})
     cpu = None if args.no_cpu_baseline else cpu_baseline(cells, ppc)
     if cpu:
         log("cpu baseline:", cpu)
@@ -234,6 +288,7 @@ def main():
             "step_breakdown_ms": {"field": field_ms, "push": push_ms, "scan": scan_ms},
         },
         "cpu_baseline": cpu,
+        "surrogate": sur,
     }
     print(json.dumps(out))
 

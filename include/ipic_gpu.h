@@ -39,6 +39,11 @@ extern "C" {
 /* flags */
 #define IPIC_FLAG_TRACK_IDS 1u    /* carry a uint32 id per particle (input order) */
 #define IPIC_FLAG_PHASE_TIMING 4u /* record per-phase HIP events (synchronises after each step) */
+#define IPIC_FLAG_REFLECT 8u      /* particles reflect at the global faces (exportParticles,
+                                     cell.h:1103-1121) instead of wrapping; fields stay periodic */
+#define IPIC_FLAG_UNIFORM_BACKGROUND 16u /* upload_fields keeps only the uniform part of the
+                                     fields as static background; the rest becomes the evolving
+                                     FP32 state (needed by the field operator) */
 
 /* Particle species: every particle of a species shares Particle::q and Particle::qom
  * (particle.h:15-16). */
@@ -124,7 +129,11 @@ int ipic_gpu_upload_particles(ipic_ctx* ctx, int32_t species, int64_t n,
 int ipic_gpu_init_particles(ipic_ctx* ctx, int32_t species, const int32_t* ppc, int32_t uniform_ppc,
                             const double vth[3], const double drift[3], uint64_t seed);
 
-/* Field (nodes) and BcField (centres) state, n^3 x 3 doubles each; NULL leaves one as is. */
+/* Field (nodes) and BcField (centres) state, n^3 x 3 doubles each; NULL leaves one as is.
+ * The uploaded state becomes a static background held in FP64 (its uniform part as
+ * constants, the rest as arrays with FP64 curl sources) plus a zero FP32 perturbation that the
+ * leapfrog evolves, so a large static field costs no precision in the curls; with
+ * IPIC_FLAG_UNIFORM_BACKGROUND only the uniform part is background. */
 int ipic_gpu_upload_fields(ipic_ctx* ctx, const double* E, const double* Bc);
 
 /* simulateSteps (simulator.h:127-242) with the intended full loop of simulator.h:189-223:
