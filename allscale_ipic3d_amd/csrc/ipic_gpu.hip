@@ -132,7 +132,10 @@ __device__ __forceinline__ int dcell(float o) {
 }
 
 __device__ __forceinline__ int wrapc(int i, int n) {
-    int r = i % n;
+    if ((unsigned)i < (unsigned)n) return i;
+    if (i < 0 && i >= -n) return i + n;
+    if (i >= n && i < 2 * n) return i - n;
+    const int r = i % n;
     return r < 0 ? r + n : r;
 }
 
@@ -207,10 +210,49 @@ __device__ __forceinline__ void tri_weights(float x, float y, float z, float wt[
     wt[6] = a11 * z0; wt[7] = a11 * z;
 }
 
+constexpr int JH = IPIC_JHALO;
+
+// Tile geometry of the particle kernel, from Geo at run time or, for 8x8x8 tiles, as
+// compile-time constants (cell splits and node indices fold into shifts and offsets).
+template <bool FIX8>
+struct TileGeo {
+    int T0, T1, T2, TC, nj1, nj2, NJ;
+    int nj0;
+    float iT2, iT12;
+    __device__ __forceinline__ explicit TileGeo(const Geo& g)
+        : T0(g.T[0]), T1(g.T[1]), T2(g.T[2]), TC(g.TC), nj1(g.nj[1]), nj2(g.nj[2]), NJ(g.NJ), nj0(g.nj[0]),
+          iT2(g.inv_T[2]), iT12(g.inv_T12) {}
+    // tile-local cell index -> coordinates
+    __device__ __forceinline__ void split(int c, int& lx, int& ly, int& lz) const {
+        lx = fdivi(c, iT12);
+        const int r = c - lx * T1 * T2;
+        ly = fdivi(r, iT2);
+        lz = r - ly * T2;
+    }
+    // wrapped global cell -> tile-major cell id
+    __device__ __forceinline__ int cid(const Geo& g, int x, int y, int z) const { return cid_of(g, x, y, z); }
+};
+
+template <>
+struct TileGeo<true> {
+    static constexpr int T0 = 8, T1 = 8, T2 = 8, TC = 512;
+    static constexpr int nj0 = 9 + 2 * JH, nj1 = nj0, nj2 = nj0, NJ = nj0 * nj0 * nj0;
+    __device__ __forceinline__ explicit TileGeo(const Geo&) {}
+    __device__ __forceinline__ void split(int c, int& lx, int& ly, int& lz) const {
+        lx = c >> 6;
+        ly = (c >> 3) & 7;
+        lz = c & 7;
+    }
+    __device__ __forceinline__ int cid(const Geo& g, int x, int y, int z) const {
+        return (((x >> 3) * g.NTl[1] + (y >> 3)) * g.NTl[2] + (z >> 3)) * 512 + ((x & 7) * 8 + (y & 7)) * 8 + (z & 7);
+    }
+};
+
 // Deposit of one particle outside the run's own cell: corner (i,j,k) of cell e goes to the
 // tile's LDS accumulator when inside its halo, else to the global overflow accumulator.
-__device__ __forceinline__ void deposit_direct(const PushArgs& A, long long* sJ, int x0, int y0, int z0,
-                                               int elx, int ely, int elz, const float wt[8], float jx,
+template <class TG>
+__device__ __forceinline__ void deposit_direct(const PushArgs& A, const TG& G, long long* sJ, int x0, int y0,
+                                               int z0, int elx, int ely, int elz, const float wt[8], float jx,
                                                float jy, float jz) {
     const Geo& g = A.g;
 #pragma unroll
@@ -220,12 +262,12 @@ __device__ __forceinline__ void deposit_direct(const PushArgs& A, long long* sJ,
 #pragma unroll
             for (int k = 0; k < 2; ++k) {
                 const float ww = wt[(i * 2 + j) * 2 + k];
-                const int nx = elx + g.jh + i, ny = ely + g.jh + j, nz = elz + g.jh + k;
-                if (nx >= 0 && nx < g.nj[0] && ny >= 0 && ny < g.nj[1] && nz >= 0 && nz < g.nj[2]) {
-                    const int l = (nx * g.nj[1] + ny) * g.nj[2] + nz;
+                const int nx = elx + JH + i, ny = ely + JH + j, nz = elz + JH + k;
+                if (nx >= 0 && nx < G.nj0 && ny >= 0 && ny < G.nj1 && nz >= 0 && nz < G.nj2) {
+                    const int l = (nx * G.nj1 + ny) * G.nj2 + nz;
                     fix_add(&sJ[l], jx * ww, g.fix_pre);
-                    fix_add(&sJ[g.NJ + l], jy * ww, g.fix_pre);
-                    fix_add(&sJ[2 * g.NJ + l], jz * ww, g.fix_pre);
+                    fix_add(&sJ[G.NJ + l], jy * ww, g.fix_pre);
+                    fix_add(&sJ[2 * G.NJ + l], jz * ww, g.fix_pre);
                 } else {
                     const int gid = nid_of(g, wrapc(x0 + elx + i, g.n[0]), wrapc(y0 + ely + j, g.n[1]),
                                            wrapc(z0 + elz + k, g.n[2]));
@@ -237,13 +279,14 @@ __device__ __forceinline__ void deposit_direct(const PushArgs& A, long long* sJ,
             }
 }
 
-__device__ __forceinline__ void count_add(const Geo& g, int* s_arr, int* gl, int x0, int y0, int z0,
+template <class TG>
+__device__ __forceinline__ void count_add(const Geo& g, const TG& G, int* s_arr, int* gl, int x0, int y0, int z0,
                                           int ex, int ey, int ez) {
     const int lx = ex - x0, ly = ey - y0, lz = ez - z0;
-    if (lx >= 0 && lx < g.T[0] && ly >= 0 && ly < g.T[1] && lz >= 0 && lz < g.T[2]) {
-        atomicAdd(&s_arr[(lx * g.T[1] + ly) * g.T[2] + lz], 1);
+    if (lx >= 0 && lx < G.T0 && ly >= 0 && ly < G.T1 && lz >= 0 && lz < G.T2) {
+        atomicAdd(&s_arr[(lx * G.T1 + ly) * G.T2 + lz], 1);
     } else {
-        atomicAdd(&gl[cid_of(g, wrapc(ex, g.n[0]), wrapc(ey, g.n[1]), wrapc(ez, g.n[2]))], 1);
+        atomicAdd(&gl[G.cid(g, wrapc(ex, g.n[0]), wrapc(ey, g.n[1]), wrapc(ez, g.n[2]))], 1);
     }
 }
 
@@ -501,11 +544,18 @@ __host__ __device__ inline int field_nodes(const Geo& g) { return (g.T[0] + 1) *
 #define IPIC_PUSH_ATTR
 #endif
 
-template <int MODE, bool TRACK>
+// option bits of the particle kernel
+enum { OPT_FIX8 = 1, OPT_SINK = 2, OPT_REFLECT = 4 };
+
+template <int MODE, bool TRACK, int OPT>
 __global__ __launch_bounds__(NTP) IPIC_PUSH_ATTR void k_particles(PushArgs A) {
     const Geo& g = A.g;
+    using TG = TileGeo<(OPT & OPT_FIX8) != 0>;
+    const TG G(g);
+    constexpr bool SINK = (OPT & OPT_SINK) != 0, REFLECT = (OPT & OPT_REFLECT) != 0;
     extern __shared__ __align__(16) unsigned char smem[];
-    const PLayout L = particle_layout(g.NJ, g.TC, MODE == MODE_PUSH ? field_nodes(g) : 0);
+    const PLayout L =
+        particle_layout(G.NJ, G.TC, MODE == MODE_PUSH ? (G.T0 + 1) * (G.T1 + 1) * (G.T2 + 1) : 0);
     long long* sJ = (long long*)smem;
     float4* sF4 = (float4*)(smem + L.off_f4);
     float2* sF2 = (float2*)(smem + L.off_f2);
@@ -522,16 +572,16 @@ __global__ __launch_bounds__(NTP) IPIC_PUSH_ATTR void k_particles(PushArgs A) {
 
     const int t = xcd_tile(blockIdx.x, g.ntiles);
     const int tx = t / (g.NTl[1] * g.NTl[2]), ty = (t / g.NTl[2]) % g.NTl[1], tz = t % g.NTl[2];
-    const int x0 = tx * g.T[0], y0 = ty * g.T[1], z0 = tz * g.T[2];
-    const int cbase = t * g.TC;
+    const int x0 = tx * G.T0, y0 = ty * G.T1, z0 = tz * G.T2;
+    const int cbase = t * G.TC;
     const float NaNf = __builtin_nanf("");
-    const int fsy = g.T[2] + 1, fsx = (g.T[1] + 1) * fsy;  // node strides of the LDS field
+    const int fsy = G.T2 + 1, fsx = (G.T1 + 1) * fsy;  // node strides of the LDS field
 
     if (MODE != MODE_REBIN)
-        for (int i = tid; i < 3 * g.NJ; i += NTP) sJ[i] = 0;
+        for (int i = tid; i < 3 * G.NJ; i += NTP) sJ[i] = 0;
     if (MODE == MODE_PUSH) {
         // the tile's (T+1)^3 nodes of E and B, background added, divided by the cell volume
-        const int nf = (g.T[0] + 1) * fsx;
+        const int nf = (G.T0 + 1) * fsx;
         for (int i = tid; i < nf; i += NTP) {
             const int ix = i / fsx, r = i - ix * fsx, iy = r / fsy, iz = r - iy * fsy;
             const int nd = wrap1(x0 + ix, g.n[0]) * g.s0 + wrap1(y0 + iy, g.n[1]) * g.n[2] + wrap1(z0 + iz, g.n[2]);
@@ -548,22 +598,22 @@ __global__ __launch_bounds__(NTP) IPIC_PUSH_ATTR void k_particles(PushArgs A) {
         const Rsrc rout = make_rsrc(S.out, S.cap * 24);
         const unsigned cs = (unsigned)S.cap * 4u;  // byte stride between the six arrays
         __syncthreads();
-        for (int i = tid; i <= g.TC; i += NTP) s_seg[i] = S.seg[cbase + i];
-        for (int i = tid; i < g.TC; i += NTP) {
+        for (int i = tid; i <= G.TC; i += NTP) s_seg[i] = S.seg[cbase + i];
+        for (int i = tid; i < G.TC; i += NTP) {
             s_cnt[i] = 0;
             s_stay[i] = 0;
         }
         __syncthreads();
         // cells of wave w: [s_wlo[w], s_wlo[w+1]), split at equal particle counts
         if (tid <= NW) {
-            const long long tot = (long long)s_seg[g.TC] - s_seg[0];
+            const long long tot = (long long)s_seg[G.TC] - s_seg[0];
             const int target = s_seg[0] + (int)(tot * tid / NW);
-            int lo = 0, hi = g.TC;  // first cell c with s_seg[c] >= target
+            int lo = 0, hi = G.TC;  // first cell c with s_seg[c] >= target
             while (lo < hi) {
                 const int mid = (lo + hi) >> 1;
                 if (s_seg[mid] >= target) hi = mid; else lo = mid + 1;
             }
-            s_wlo[tid] = (tid == 0) ? 0 : (tid == NW ? g.TC : lo);
+            s_wlo[tid] = (tid == 0) ? 0 : (tid == NW ? G.TC : lo);
         }
         __syncthreads();
         const int chi = s_wlo[wid + 1];
@@ -636,10 +686,8 @@ __global__ __launch_bounds__(NTP) IPIC_PUSH_ATTR void k_particles(PushArgs A) {
                 }
             }
             const int c = c_cur + k;
-            const int lx = fdivi(c, g.inv_T12);
-            const int crem = c - lx * g.T[1] * g.T[2];
-            const int ly = fdivi(crem, g.inv_T[2]);
-            const int lz = crem - ly * g.T[2];
+            int lx, ly, lz;
+            G.split(c, lx, ly, lz);
             const int ax = x0 + lx, ay = y0 + ly, az = z0 + lz;
             const int nb = lx * fsx + ly * fsy + lz;  // the run cell's first node in the LDS field
 
@@ -678,9 +726,9 @@ __global__ __launch_bounds__(NTP) IPIC_PUSH_ATTR void k_particles(PushArgs A) {
                             float Ex, Ey, Ez, Bx, By, Bz;
                             gather_lds(sF4, sF2, nb, fsx, fsy, px, py, pz, Ex, Ey, Ez, Bx, By, Bz);
                             boris_move(g, qom, subk, Ex, Ey, Ez, Bx, By, Bz, vx, vy, vz, px, py, pz);
-                            if (g.reflect) reflect_faces(g, ax, ay, az, px, py, pz, vx, vy, vz);
+                            if (REFLECT) reflect_faces(g, ax, ay, az, px, py, pz, vx, vy, vz);
                             sunk = !finite3(px, py, pz);
-                            if (g.sink && !sunk) {
+                            if (SINK && !sunk) {
                                 const int dex = dcell(px), dey = dcell(py), dez = dcell(pz);
                                 const float sx_ = g.origin[0] + ((float)wrapc(ax + dex, g.n[0]) + px - (float)dex) * g.w[0] - g.sink_c[0];
                                 const float sy_ = g.origin[1] + ((float)wrapc(ay + dey, g.n[1]) + py - (float)dey) * g.w[1] - g.sink_c[1];
@@ -734,8 +782,9 @@ __global__ __launch_bounds__(NTP) IPIC_PUSH_ATTR void k_particles(PushArgs A) {
                         const int dex = dcell(px), dey = dcell(py), dez = dcell(pz);
                         float wt[8];
                         tri_weights(px - (float)dex, py - (float)dey, pz - (float)dez, wt);
-                        deposit_direct(A, sJ, x0, y0, z0, lx + dex, ly + dey, lz + dez, wt, qs * vx, qs * vy, qs * vz);
-                        count_add(g, s_cnt, S.cnt_acc, x0, y0, z0, ax + dex, ay + dey, az + dez);
+                        deposit_direct(A, G, sJ, x0, y0, z0, lx + dex, ly + dey, lz + dez, wt, qs * vx, qs * vy,
+                                       qs * vz);
+                        count_add(g, G, s_cnt, S.cnt_acc, x0, y0, z0, ax + dex, ay + dey, az + dez);
                     }
                 }
                 kn += __popcll(bk);
@@ -751,8 +800,8 @@ __global__ __launch_bounds__(NTP) IPIC_PUSH_ATTR void k_particles(PushArgs A) {
                         for (int j = 0; j < 2; ++j)
 #pragma unroll
                             for (int kk = 0; kk < 2; ++kk) {
-                                const int l = ((lx + g.jh + i) * g.nj[1] + (ly + g.jh + j)) * g.nj[2] + (lz + g.jh + kk);
-                                fix_add(&sJ[d * g.NJ + l], N[(i * 2 + j) * 2 + kk], g.fix_pre);
+                                const int l = ((lx + JH + i) * G.nj1 + (ly + JH + j)) * G.nj2 + (lz + JH + kk);
+                                fix_add(&sJ[d * G.NJ + l], N[(i * 2 + j) * 2 + kk], g.fix_pre);
                             }
                 }
             }
@@ -768,25 +817,21 @@ __global__ __launch_bounds__(NTP) IPIC_PUSH_ATTR void k_particles(PushArgs A) {
                 const float* e = K0 + qi * 8;
                 const float px = e[0], py = e[1], pz = e[2], vx = e[3], vy = e[4], vz = e[5];
                 const int kc = c_cur + __float_as_int(e[6]);
-                const int qlx = fdivi(kc, g.inv_T12);
-                const int qr = kc - qlx * g.T[1] * g.T[2];
-                const int qly = fdivi(qr, g.inv_T[2]);
-                const int qlz = qr - qly * g.T[2];
+                int qlx, qly, qlz;
+                G.split(kc, qlx, qly, qlz);
                 const int dex = dcell(px), dey = dcell(py), dez = dcell(pz);
                 float wt[8];
                 tri_weights(px - (float)dex, py - (float)dey, pz - (float)dez, wt);
-                deposit_direct(A, sJ, x0, y0, z0, qlx + dex, qly + dey, qlz + dez, wt, qs * vx, qs * vy, qs * vz);
-                count_add(g, s_cnt, S.cnt_acc, x0, y0, z0, x0 + qlx + dex, y0 + qly + dey, z0 + qlz + dez);
+                deposit_direct(A, G, sJ, x0, y0, z0, qlx + dex, qly + dey, qlz + dez, wt, qs * vx, qs * vy, qs * vz);
+                count_add(g, G, s_cnt, S.cnt_acc, x0, y0, z0, x0 + qlx + dex, y0 + qly + dey, z0 + qlz + dez);
             }
             // particles sitting in another cell b: full processing, one per lane
             for (int qi = lane; qi < qn; qi += WS) {
                 const int ent = Q[qi];
                 const int inq = plo + (ent & 511);
                 const int kc = c_cur + (ent >> 9);
-                const int qlx = fdivi(kc, g.inv_T12);
-                const int qr = kc - qlx * g.T[1] * g.T[2];
-                const int qly = fdivi(qr, g.inv_T[2]);
-                const int qlz = qr - qly * g.T[2];
+                int qlx, qly, qlz;
+                G.split(kc, qlx, qly, qlz);
                 const unsigned vq = (unsigned)inq * 4u;
                 const float ox = ldp(rin, vq, 0), oy = ldp(rin, vq, cs), oz = ldp(rin, vq, 2 * cs);
                 float vx = ldp(rin, vq, 3 * cs), vy = ldp(rin, vq, 4 * cs), vz = ldp(rin, vq, 5 * cs);
@@ -799,18 +844,18 @@ __global__ __launch_bounds__(NTP) IPIC_PUSH_ATTR void k_particles(PushArgs A) {
                 int dex = 0, dey = 0, dez = 0;
                 if (MODE == MODE_PUSH) {
                     float Ex, Ey, Ez, Bx, By, Bz;
-                    if ((unsigned)blx < (unsigned)g.T[0] && (unsigned)bly < (unsigned)g.T[1] &&
-                        (unsigned)blz < (unsigned)g.T[2])
+                    if ((unsigned)blx < (unsigned)G.T0 && (unsigned)bly < (unsigned)G.T1 &&
+                        (unsigned)blz < (unsigned)G.T2)
                         gather_lds(sF4, sF2, blx * fsx + bly * fsy + blz, fsx, fsy, px, py, pz, Ex, Ey, Ez, Bx, By, Bz);
                     else
                         gather_direct(A, bxw, byw, bzw, px, py, pz, Ex, Ey, Ez, Bx, By, Bz);
                     boris_move(g, qom, subk, Ex, Ey, Ez, Bx, By, Bz, vx, vy, vz, px, py, pz);
-                    if (g.reflect) reflect_faces(g, bxw, byw, bzw, px, py, pz, vx, vy, vz);
+                    if (REFLECT) reflect_faces(g, bxw, byw, bzw, px, py, pz, vx, vy, vz);
                     sunk = !finite3(px, py, pz);
                     if (!sunk) {
                         dex = dcell(px); dey = dcell(py); dez = dcell(pz);
                     }
-                    if (g.sink && !sunk) {
+                    if (SINK && !sunk) {
                         const float rx_ = g.origin[0] + ((float)wrapc(bx + dex, g.n[0]) + px - (float)dex) * g.w[0] - g.sink_c[0];
                         const float ry_ = g.origin[1] + ((float)wrapc(by + dey, g.n[1]) + py - (float)dey) * g.w[1] - g.sink_c[1];
                         const float rz_ = g.origin[2] + ((float)wrapc(bz + dez, g.n[2]) + pz - (float)dez) * g.w[2] - g.sink_c[2];
@@ -819,7 +864,7 @@ __global__ __launch_bounds__(NTP) IPIC_PUSH_ATTR void k_particles(PushArgs A) {
                 }
                 bool ok = true;
                 if (MODE != MODE_DEPOSIT) {
-                    const int bc = cid_of(g, bxw, byw, bzw);
+                    const int bc = G.cid(g, bxw, byw, bzw);
                     const int oidx = S.start[bc] + S.stay[bc] + atomicAdd(&S.arr[bc], 1);
                     if (oidx < 0 || oidx >= S.start[bc + 1] || (long long)oidx >= S.cap) {
                         atomicOr(A.err, 1);
@@ -834,11 +879,11 @@ __global__ __launch_bounds__(NTP) IPIC_PUSH_ATTR void k_particles(PushArgs A) {
                 } else if (MODE != MODE_REBIN && ok) {
                     float wt[8];
                     tri_weights(px - (float)dex, py - (float)dey, pz - (float)dez, wt);
-                    deposit_direct(A, sJ, x0, y0, z0, blx + dex, bly + dey, blz + dez, wt, qs * vx, qs * vy,
+                    deposit_direct(A, G, sJ, x0, y0, z0, blx + dex, bly + dey, blz + dez, wt, qs * vx, qs * vy,
                                    qs * vz);
                     if (MODE == MODE_PUSH) {
-                        count_add(g, s_cnt, S.cnt_acc, x0, y0, z0, bx + dex, by + dey, bz + dez);
-                        if ((dex | dey | dez) == 0) count_add(g, s_stay, S.stay_acc, x0, y0, z0, bx, by, bz);
+                        count_add(g, G, s_cnt, S.cnt_acc, x0, y0, z0, bx + dex, by + dey, bz + dez);
+                        if ((dex | dey | dez) == 0) count_add(g, G, s_stay, S.stay_acc, x0, y0, z0, bx, by, bz);
                     }
                 }
             }
@@ -863,7 +908,7 @@ __global__ __launch_bounds__(NTP) IPIC_PUSH_ATTR void k_particles(PushArgs A) {
         }
         __syncthreads();
         if (MODE == MODE_PUSH) {
-            for (int i = tid; i < g.TC; i += NTP) {
+            for (int i = tid; i < G.TC; i += NTP) {
                 if (s_cnt[i]) atomicAdd(&S.cnt_acc[cbase + i], s_cnt[i]);
                 if (s_stay[i]) atomicAdd(&S.stay_acc[cbase + i], s_stay[i]);
             }
@@ -871,8 +916,8 @@ __global__ __launch_bounds__(NTP) IPIC_PUSH_ATTR void k_particles(PushArgs A) {
     }
     __syncthreads();
     if (MODE != MODE_REBIN) {
-        float* dst = A.jpart + (long long)t * 3 * g.NJ;
-        for (int i = tid; i < 3 * g.NJ; i += NTP) dst[i] = (float)((double)sJ[i] * g.fix_inv);
+        float* dst = A.jpart + (long long)t * 3 * G.NJ;
+        for (int i = tid; i < 3 * G.NJ; i += NTP) dst[i] = (float)((double)sJ[i] * g.fix_inv);
     }
 }
 
@@ -980,9 +1025,9 @@ __global__ __launch_bounds__(256) void k_field_E(FieldArgs F, int from_partials)
         Jx = j4.x; Jy = j4.y; Jz = j4.z;
     }
     const float4 b = F.Bc[i];
-    const float4 bmx = F.Bc[nid_of(g, wrapc(x - 1, g.n[0]), y, z)];
-    const float4 bmy = F.Bc[nid_of(g, x, wrapc(y - 1, g.n[1]), z)];
-    const float4 bmz = F.Bc[nid_of(g, x, y, wrapc(z - 1, g.n[2]))];
+    const float4 bmx = F.Bc[nid_of(g, wrap1(x - 1, g.n[0]), y, z)];
+    const float4 bmy = F.Bc[nid_of(g, x, wrap1(y - 1, g.n[1]), z)];
+    const float4 bmz = F.Bc[nid_of(g, x, y, wrap1(z - 1, g.n[2]))];
     float4 e = F.E[i];
     const float dt = F.dt, c2 = F.c2;
     e.x = e.x + dt * (c2 * ((b.z - bmy.z) * g.inv_w[1] - (b.y - bmz.y) * g.inv_w[2]) - Jx);
@@ -1008,9 +1053,9 @@ __global__ __launch_bounds__(256) void k_field_B(FieldArgs F) {
     int x, y, z;
     xyz_of(g, i, x, y, z);
     const float4 e = F.E[i];
-    const float4 epx = F.E[nid_of(g, wrapc(x + 1, g.n[0]), y, z)];
-    const float4 epy = F.E[nid_of(g, x, wrapc(y + 1, g.n[1]), z)];
-    const float4 epz = F.E[nid_of(g, x, y, wrapc(z + 1, g.n[2]))];
+    const float4 epx = F.E[nid_of(g, wrap1(x + 1, g.n[0]), y, z)];
+    const float4 epy = F.E[nid_of(g, x, wrap1(y + 1, g.n[1]), z)];
+    const float4 epz = F.E[nid_of(g, x, y, wrap1(z + 1, g.n[2]))];
     float4 b = F.Bc[i];
     const float dt = F.dt;
     b.x = b.x - dt * ((epy.z - e.z) * g.inv_w[1] - (epz.y - e.y) * g.inv_w[2]);
@@ -1037,7 +1082,7 @@ __global__ __launch_bounds__(256) void k_c2n(FieldArgs F) {
         for (int b = 0; b < 2; ++b)
 #pragma unroll
             for (int c = 0; c < 2; ++c) {
-                const float4 v = F.Bc[nid_of(g, wrapc(x - a, g.n[0]), wrapc(y - b, g.n[1]), wrapc(z - c, g.n[2]))];
+                const float4 v = F.Bc[nid_of(g, wrap1(x - a, g.n[0]), wrap1(y - b, g.n[1]), wrap1(z - c, g.n[2]))];
                 sx += v.x; sy += v.y; sz += v.z;
             }
     float4 r = make_float4(0.125f * sx, 0.125f * sy, 0.125f * sz, 0.0f);
@@ -1278,9 +1323,9 @@ __global__ __launch_bounds__(256) void k_reduce_fields(FieldArgs F, double* out)
         sb += b[0] * b[0] + b[1] * b[1] + b[2] * b[2];
         sej += e[0] * j.x + e[1] * j.y + e[2] * j.z;
         double bpx[3], bpy[3], bpz[3];
-        tot_field(F.Bc, F.Bcbg, g.B0, nid_of(g, wrapc(x + 1, g.n[0]), y, z), bpx);
-        tot_field(F.Bc, F.Bcbg, g.B0, nid_of(g, x, wrapc(y + 1, g.n[1]), z), bpy);
-        tot_field(F.Bc, F.Bcbg, g.B0, nid_of(g, x, y, wrapc(z + 1, g.n[2])), bpz);
+        tot_field(F.Bc, F.Bcbg, g.B0, nid_of(g, wrap1(x + 1, g.n[0]), y, z), bpx);
+        tot_field(F.Bc, F.Bcbg, g.B0, nid_of(g, x, wrap1(y + 1, g.n[1]), z), bpy);
+        tot_field(F.Bc, F.Bcbg, g.B0, nid_of(g, x, y, wrap1(z + 1, g.n[2])), bpz);
         const double dv = (bpx[0] - b[0]) * g.inv_w[0] + (bpy[1] - b[1]) * g.inv_w[1] +
                           (bpz[2] - b[2]) * g.inv_w[2];
         sd2 += dv * dv;
@@ -1314,8 +1359,8 @@ __global__ __launch_bounds__(256) void k_op_check(FieldArgs F, const float4* Eol
         wn += nx * nx + ny * ny + nz * nz + F.c2 * ((double)bn.x * bn.x + (double)bn.y * bn.y + (double)bn.z * bn.z);
         ejo += ox * j.x + oy * j.y + oz * j.z;
         ejn += nx * j.x + ny * j.y + nz * j.z;
-        const int ipx = nid_of(g, wrapc(x + 1, g.n[0]), y, z), ipy = nid_of(g, x, wrapc(y + 1, g.n[1]), z),
-                  ipz = nid_of(g, x, y, wrapc(z + 1, g.n[2]));
+        const int ipx = nid_of(g, wrap1(x + 1, g.n[0]), y, z), ipy = nid_of(g, x, wrap1(y + 1, g.n[1]), z),
+                  ipz = nid_of(g, x, y, wrap1(z + 1, g.n[2]));
         const double d0x = (double)bn.x - bo.x, d0y = (double)bn.y - bo.y, d0z = (double)bn.z - bo.z;
         const double dpx = (double)F.Bc[ipx].x - Bold[ipx].x;
         const double dpy = (double)F.Bc[ipy].y - Bold[ipy].y;
@@ -1569,6 +1614,26 @@ ScanArgs scan_args(ipic_ctx* ctx) {
     return A;
 }
 
+template <int MODE, bool TRACK, int OPT>
+void launch_k(dim3 grid, dim3 block, size_t lds, hipStream_t st, const PushArgs& A) {
+    hipLaunchKernelGGL((k_particles<MODE, TRACK, OPT>), grid, block, lds, st, A);
+}
+
+template <int MODE, bool TRACK>
+void launch_opt(int opt, dim3 grid, dim3 block, size_t lds, hipStream_t st, const PushArgs& A) {
+    if (MODE != MODE_PUSH) opt &= OPT_FIX8;  // sink and reflection act in the push only
+    switch (opt) {
+        case 0: launch_k<MODE, TRACK, 0>(grid, block, lds, st, A); break;
+        case 1: launch_k<MODE, TRACK, 1>(grid, block, lds, st, A); break;
+        case 2: launch_k<MODE, TRACK, (MODE == MODE_PUSH ? 2 : 0)>(grid, block, lds, st, A); break;
+        case 3: launch_k<MODE, TRACK, (MODE == MODE_PUSH ? 3 : 1)>(grid, block, lds, st, A); break;
+        case 4: launch_k<MODE, TRACK, (MODE == MODE_PUSH ? 4 : 0)>(grid, block, lds, st, A); break;
+        case 5: launch_k<MODE, TRACK, (MODE == MODE_PUSH ? 5 : 1)>(grid, block, lds, st, A); break;
+        case 6: launch_k<MODE, TRACK, (MODE == MODE_PUSH ? 6 : 0)>(grid, block, lds, st, A); break;
+        default: launch_k<MODE, TRACK, (MODE == MODE_PUSH ? 7 : 1)>(grid, block, lds, st, A); break;
+    }
+}
+
 int launch_particles(ipic_ctx* ctx, int mode) {
     if (mode != MODE_REBIN && !ctx->j_summed) {
         const long long n = 3LL * ctx->g.ncell;
@@ -1577,17 +1642,20 @@ int launch_particles(ipic_ctx* ctx, int mode) {
         hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(1), 0, ctx->stream, ctx->jflag);
     }
     PushArgs A = push_args(ctx);
-    dim3 grid(ctx->g.ntiles), block(NTP);
-    const size_t lds =
-        (size_t)particle_layout(ctx->g.NJ, ctx->g.TC, mode == MODE_PUSH ? field_nodes(ctx->g) : 0).total;
+    const Geo& g = ctx->g;
+    dim3 grid(g.ntiles), block(NTP);
+    const size_t lds = (size_t)particle_layout(g.NJ, g.TC, mode == MODE_PUSH ? field_nodes(g) : 0).total;
+    const int opt = ((g.T[0] == 8 && g.T[1] == 8 && g.T[2] == 8) ? OPT_FIX8 : 0) | (g.sink ? OPT_SINK : 0) |
+                    (g.reflect ? OPT_REFLECT : 0);
+    hipStream_t st = ctx->stream;
     if (ctx->track) {
-        if (mode == MODE_PUSH) hipLaunchKernelGGL((k_particles<MODE_PUSH, true>), grid, block, lds, ctx->stream, A);
-        else if (mode == MODE_DEPOSIT) hipLaunchKernelGGL((k_particles<MODE_DEPOSIT, true>), grid, block, lds, ctx->stream, A);
-        else hipLaunchKernelGGL((k_particles<MODE_REBIN, true>), grid, block, lds, ctx->stream, A);
+        if (mode == MODE_PUSH) launch_opt<MODE_PUSH, true>(opt, grid, block, lds, st, A);
+        else if (mode == MODE_DEPOSIT) launch_opt<MODE_DEPOSIT, true>(opt, grid, block, lds, st, A);
+        else launch_opt<MODE_REBIN, true>(opt, grid, block, lds, st, A);
     } else {
-        if (mode == MODE_PUSH) hipLaunchKernelGGL((k_particles<MODE_PUSH, false>), grid, block, lds, ctx->stream, A);
-        else if (mode == MODE_DEPOSIT) hipLaunchKernelGGL((k_particles<MODE_DEPOSIT, false>), grid, block, lds, ctx->stream, A);
-        else hipLaunchKernelGGL((k_particles<MODE_REBIN, false>), grid, block, lds, ctx->stream, A);
+        if (mode == MODE_PUSH) launch_opt<MODE_PUSH, false>(opt, grid, block, lds, st, A);
+        else if (mode == MODE_DEPOSIT) launch_opt<MODE_DEPOSIT, false>(opt, grid, block, lds, st, A);
+        else launch_opt<MODE_REBIN, false>(opt, grid, block, lds, st, A);
     }
     HIPCK(ctx, hipGetLastError());
     if (mode != MODE_REBIN) ctx->j_summed = false;
