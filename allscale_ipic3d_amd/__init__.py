@@ -19,6 +19,7 @@ FLAG_TRACK_IDS = 1
 FLAG_PHASE_TIMING = 4
 FLAG_REFLECT = 8
 FLAG_UNIFORM_BACKGROUND = 16
+FLAG_DIPOLE_MOVER = 32
 
 
 class SpeciesDesc(C.Structure):
@@ -40,6 +41,7 @@ class Desc(C.Structure):
         ("sink_radius", C.c_double),
         ("tile", C.c_int32 * 3),
         ("flags", C.c_uint32),
+        ("dipole_moment", C.c_double),
     ]
 
 
@@ -182,7 +184,7 @@ class Simulation:
 
     def __init__(self, n, cell_width=(1.0, 1.0, 1.0), origin=(0.0, 0.0, 0.0), dt=0.1, c=1.0,
                  species=((-1.0, -1.0),), field_substeps=1, sink_center=None, sink_radius=0.0,
-                 tile=(0, 0, 0), track_ids=False, flags=0):
+                 tile=(0, 0, 0), track_ids=False, flags=0, dipole_moment=None):
         L = load_library()
         if np.isscalar(n):
             n = (n, n, n)
@@ -202,6 +204,9 @@ class Simulation:
             d.sink_radius = float(sink_radius)
         d.tile[:] = [int(v) for v in tile]
         d.flags = int(flags) | (FLAG_TRACK_IDS if track_ids else 0)
+        if dipole_moment is not None:  # the mover as shipped: analytic dipole, E = 0
+            d.flags |= FLAG_DIPOLE_MOVER
+            d.dipole_moment = float(dipole_moment)
         self.desc = d
         self.n = tuple(int(v) for v in n)
         self.ncell = self.n[0] * self.n[1] * self.n[2]

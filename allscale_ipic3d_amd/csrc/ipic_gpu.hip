@@ -78,6 +78,7 @@ struct Geo {
     float pi_c;
     int sink;
     int reflect;          // 1: reflect at the global faces (cell.h:1103-1121) instead of wrapping
+    float dip_m;          // -B1z R^3 of the mover as shipped (analytic dipole at the origin)
     float sink_c[3];
     float sink_r2;
     float E0[3];          // uniform part of the background field (added in the gather)
@@ -420,6 +421,21 @@ __device__ __forceinline__ void reflect_faces(const Geo& g, int cx, int cy, int 
     }
 }
 
+// The mover as shipped (cell.h:919-945): E = 0 and the field of a z dipole at the coordinate
+// origin, B = m (3xz, 3yz, 2z^2 - x^2 - y^2) / |x|^5 with m = -B1z R^3, at the absolute
+// position of offsets (px,py,pz) in cell (cx,cy,cz).
+__device__ __forceinline__ void dipole_field(const Geo& g, int cx, int cy, int cz, float px, float py, float pz,
+                                             float& Bx, float& By, float& Bz) {
+    const float x = g.origin[0] + ((float)cx + px) * g.w[0];
+    const float y = g.origin[1] + ((float)cy + py) * g.w[1];
+    const float z = g.origin[2] + ((float)cz + pz) * g.w[2];
+    const float r2 = x * x + y * y + z * z;
+    const float fac = g.dip_m / (r2 * r2 * sqrtf(r2));
+    Bx = 3.0f * x * z * fac;
+    By = 3.0f * y * z * fac;
+    Bz = (2.0f * z * z - x * x - y * y) * fac;
+}
+
 // E and B (gather arrays plus the uniform background, divided by the cell volume) at offsets
 // (x,y,z) of the cell with wrapped coordinates (cx,cy,cz), read straight from the node arrays.
 __device__ __forceinline__ void gather_direct(const PushArgs& A, int cx, int cy, int cz, float x, float y,
@@ -545,7 +561,7 @@ __host__ __device__ inline int field_nodes(const Geo& g) { return (g.T[0] + 1) *
 #endif
 
 // option bits of the particle kernel
-enum { OPT_FIX8 = 1, OPT_SINK = 2, OPT_REFLECT = 4 };
+enum { OPT_FIX8 = 1, OPT_SINK = 2, OPT_REFLECT = 4, OPT_DIPOLE = 8 };
 
 template <int MODE, bool TRACK, int OPT>
 __global__ __launch_bounds__(NTP) IPIC_PUSH_ATTR void k_particles(PushArgs A) {
@@ -553,9 +569,10 @@ __global__ __launch_bounds__(NTP) IPIC_PUSH_ATTR void k_particles(PushArgs A) {
     using TG = TileGeo<(OPT & OPT_FIX8) != 0>;
     const TG G(g);
     constexpr bool SINK = (OPT & OPT_SINK) != 0, REFLECT = (OPT & OPT_REFLECT) != 0;
+    constexpr bool DIPOLE = (OPT & OPT_DIPOLE) != 0;  // analytic dipole B, E = 0: no field staging
     extern __shared__ __align__(16) unsigned char smem[];
     const PLayout L =
-        particle_layout(G.NJ, G.TC, MODE == MODE_PUSH ? (G.T0 + 1) * (G.T1 + 1) * (G.T2 + 1) : 0);
+        particle_layout(G.NJ, G.TC, (MODE == MODE_PUSH && !DIPOLE) ? (G.T0 + 1) * (G.T1 + 1) * (G.T2 + 1) : 0);
     long long* sJ = (long long*)smem;
     float4* sF4 = (float4*)(smem + L.off_f4);
     float2* sF2 = (float2*)(smem + L.off_f2);
@@ -579,7 +596,7 @@ __global__ __launch_bounds__(NTP) IPIC_PUSH_ATTR void k_particles(PushArgs A) {
 
     if (MODE != MODE_REBIN)
         for (int i = tid; i < 3 * G.NJ; i += NTP) sJ[i] = 0;
-    if (MODE == MODE_PUSH) {
+    if (MODE == MODE_PUSH && !DIPOLE) {
         // the tile's (T+1)^3 nodes of E and B, background added, divided by the cell volume
         const int nf = (G.T0 + 1) * fsx;
         for (int i = tid; i < nf; i += NTP) {
@@ -724,7 +741,12 @@ __global__ __launch_bounds__(NTP) IPIC_PUSH_ATTR void k_particles(PushArgs A) {
                         bool sunk = false;
                         if (MODE == MODE_PUSH) {
                             float Ex, Ey, Ez, Bx, By, Bz;
-                            gather_lds(sF4, sF2, nb, fsx, fsy, px, py, pz, Ex, Ey, Ez, Bx, By, Bz);
+                            if (DIPOLE) {
+                                Ex = Ey = Ez = 0.0f;
+                                dipole_field(g, ax, ay, az, px, py, pz, Bx, By, Bz);
+                            } else {
+                                gather_lds(sF4, sF2, nb, fsx, fsy, px, py, pz, Ex, Ey, Ez, Bx, By, Bz);
+                            }
                             boris_move(g, qom, subk, Ex, Ey, Ez, Bx, By, Bz, vx, vy, vz, px, py, pz);
                             if (REFLECT) reflect_faces(g, ax, ay, az, px, py, pz, vx, vy, vz);
                             sunk = !finite3(px, py, pz);
@@ -844,8 +866,11 @@ __global__ __launch_bounds__(NTP) IPIC_PUSH_ATTR void k_particles(PushArgs A) {
                 int dex = 0, dey = 0, dez = 0;
                 if (MODE == MODE_PUSH) {
                     float Ex, Ey, Ez, Bx, By, Bz;
-                    if ((unsigned)blx < (unsigned)G.T0 && (unsigned)bly < (unsigned)G.T1 &&
-                        (unsigned)blz < (unsigned)G.T2)
+                    if (DIPOLE) {
+                        Ex = Ey = Ez = 0.0f;
+                        dipole_field(g, bxw, byw, bzw, px, py, pz, Bx, By, Bz);
+                    } else if ((unsigned)blx < (unsigned)G.T0 && (unsigned)bly < (unsigned)G.T1 &&
+                               (unsigned)blz < (unsigned)G.T2)
                         gather_lds(sF4, sF2, blx * fsx + bly * fsy + blz, fsx, fsy, px, py, pz, Ex, Ey, Ez, Bx, By, Bz);
                     else
                         gather_direct(A, bxw, byw, bzw, px, py, pz, Ex, Ey, Ez, Bx, By, Bz);
@@ -1599,7 +1624,7 @@ FieldArgs field_args(ipic_ctx* ctx) {
     F.jover = ctx->jover;
     F.jflag = ctx->jflag;
     F.jpairs = ctx->jpairs;
-    F.dt = (float)(ctx->d.dt / ctx->d.field_substeps);
+    F.dt = ctx->d.field_substeps > 0 ? (float)(ctx->d.dt / ctx->d.field_substeps) : 0.0f;
     F.c2 = (float)(ctx->d.c * ctx->d.c);
     return F;
 }
@@ -1619,18 +1644,25 @@ void launch_k(dim3 grid, dim3 block, size_t lds, hipStream_t st, const PushArgs&
     hipLaunchKernelGGL((k_particles<MODE, TRACK, OPT>), grid, block, lds, st, A);
 }
 
+// options of the push (tile shape, sink, reflection, dipole mover); the deposit and rebin
+// passes depend on the tile shape only
+template <int MODE, bool TRACK, int OPT>
+void launch_opt_t(int opt, dim3 grid, dim3 block, size_t lds, hipStream_t st, const PushArgs& A) {
+    if constexpr (OPT == 15) {
+        launch_k<MODE, TRACK, 15>(grid, block, lds, st, A);
+    } else {
+        if (opt == OPT) launch_k<MODE, TRACK, OPT>(grid, block, lds, st, A);
+        else launch_opt_t<MODE, TRACK, OPT + 1>(opt, grid, block, lds, st, A);
+    }
+}
+
 template <int MODE, bool TRACK>
 void launch_opt(int opt, dim3 grid, dim3 block, size_t lds, hipStream_t st, const PushArgs& A) {
-    if (MODE != MODE_PUSH) opt &= OPT_FIX8;  // sink and reflection act in the push only
-    switch (opt) {
-        case 0: launch_k<MODE, TRACK, 0>(grid, block, lds, st, A); break;
-        case 1: launch_k<MODE, TRACK, 1>(grid, block, lds, st, A); break;
-        case 2: launch_k<MODE, TRACK, (MODE == MODE_PUSH ? 2 : 0)>(grid, block, lds, st, A); break;
-        case 3: launch_k<MODE, TRACK, (MODE == MODE_PUSH ? 3 : 1)>(grid, block, lds, st, A); break;
-        case 4: launch_k<MODE, TRACK, (MODE == MODE_PUSH ? 4 : 0)>(grid, block, lds, st, A); break;
-        case 5: launch_k<MODE, TRACK, (MODE == MODE_PUSH ? 5 : 1)>(grid, block, lds, st, A); break;
-        case 6: launch_k<MODE, TRACK, (MODE == MODE_PUSH ? 6 : 0)>(grid, block, lds, st, A); break;
-        default: launch_k<MODE, TRACK, (MODE == MODE_PUSH ? 7 : 1)>(grid, block, lds, st, A); break;
+    if constexpr (MODE != MODE_PUSH) {
+        if (opt & OPT_FIX8) launch_k<MODE, TRACK, OPT_FIX8>(grid, block, lds, st, A);
+        else launch_k<MODE, TRACK, 0>(grid, block, lds, st, A);
+    } else {
+        launch_opt_t<MODE, TRACK, 0>(opt, grid, block, lds, st, A);
     }
 }
 
@@ -1644,9 +1676,11 @@ int launch_particles(ipic_ctx* ctx, int mode) {
     PushArgs A = push_args(ctx);
     const Geo& g = ctx->g;
     dim3 grid(g.ntiles), block(NTP);
-    const size_t lds = (size_t)particle_layout(g.NJ, g.TC, mode == MODE_PUSH ? field_nodes(g) : 0).total;
+    const bool dip = (ctx->d.flags & IPIC_FLAG_DIPOLE_MOVER) != 0;
+    const size_t lds =
+        (size_t)particle_layout(g.NJ, g.TC, (mode == MODE_PUSH && !dip) ? field_nodes(g) : 0).total;
     const int opt = ((g.T[0] == 8 && g.T[1] == 8 && g.T[2] == 8) ? OPT_FIX8 : 0) | (g.sink ? OPT_SINK : 0) |
-                    (g.reflect ? OPT_REFLECT : 0);
+                    (g.reflect ? OPT_REFLECT : 0) | (dip ? OPT_DIPOLE : 0);
     hipStream_t st = ctx->stream;
     if (ctx->track) {
         if (mode == MODE_PUSH) launch_opt<MODE_PUSH, true>(opt, grid, block, lds, st, A);
@@ -2026,7 +2060,10 @@ int ipic_gpu_create(const ipic_desc* desc, ipic_ctx** out) {
 
     ipic_ctx* ctx = new ipic_ctx();
     ctx->d = d;
-    if (ctx->d.field_substeps < 1) ctx->d.field_substeps = 1;
+    if (ctx->d.field_substeps < 0) {
+        delete ctx;
+        return fail(nullptr, IPIC_E_ARG, "field_substeps must be >= 0 (0: static field)");
+    }
     ctx->track = (d.flags & IPIC_FLAG_TRACK_IDS) != 0;
     Geo& g = ctx->g;
     for (int k = 0; k < 3; ++k) {
@@ -2070,6 +2107,7 @@ int ipic_gpu_create(const ipic_desc* desc, ipic_ctx** out) {
     }
     g.sink = d.sink ? 1 : 0;
     g.reflect = (d.flags & IPIC_FLAG_REFLECT) ? 1 : 0;
+    g.dip_m = (float)d.dipole_moment;
     g.sink_r2 = (float)(d.sink_radius * d.sink_radius);
 
     std::vector<int> jp;
@@ -2446,7 +2484,7 @@ int ipic_gpu_set_field_operator(ipic_ctx* ctx, const ipic_field_operator* op) {
         return 0;
     }
     if (!op->params) return fail(ctx, IPIC_E_ARG, "null operator parameters");
-    if (op->substeps != ctx->d.field_substeps)
+    if (op->substeps < 1 || op->substeps != ctx->d.field_substeps)
         return fail(ctx, IPIC_E_ARG, "operator substeps differ from the context's field_substeps");
     if (!(op->in_E > 0) || !(op->in_B > 0) || !(op->in_J > 0) || !std::isfinite(op->out_E) || !std::isfinite(op->out_P))
         return fail(ctx, IPIC_E_ARG, "operator scales must be positive and finite");

@@ -56,7 +56,10 @@ struct UniverseProperties {
     double planetRadius = 0.0;       // > 0: particles within it of objectCenter are removed
     Vector3 objectCenter{0.0, 0.0, 0.0};
     Vector3 origin{0.0, 0.0, 0.0};
-    int fieldSubsteps = 1;           // leapfrog substeps per particle step (J held fixed)
+    Vector3 externalMagneticField{0.0, 0.0, 0.0};
+    bool shippedDipoleMover = false; // the mover as shipped (cell.h:919-945): analytic dipole of
+                                     // externalMagneticField.z and planetRadius at the origin, E = 0
+    int fieldSubsteps = 1;           // leapfrog substeps per particle step (J held fixed); 0: static
     Coordinate3 tile{0, 0, 0};       // cells per GPU tile, 0: automatic
     std::uint32_t flags = 0;         // IPIC_FLAG_*
 };
@@ -113,6 +116,10 @@ class Universe {
         d.sink = props.planetRadius > 0.0 ? 1 : 0;
         d.sink_radius = props.planetRadius;
         d.flags = props.flags;
+        if (props.shippedDipoleMover) {
+            d.flags |= IPIC_FLAG_DIPOLE_MOVER;
+            d.dipole_moment = -props.externalMagneticField[2] * props.planetRadius * props.planetRadius * props.planetRadius;
+        }
         check(nullptr, ipic_gpu_create(&d, &ctx_));
     }
     ~Universe() {

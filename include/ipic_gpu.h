@@ -41,6 +41,9 @@ extern "C" {
 #define IPIC_FLAG_PHASE_TIMING 4u /* record per-phase HIP events (synchronises after each step) */
 #define IPIC_FLAG_REFLECT 8u      /* particles reflect at the global faces (exportParticles,
                                      cell.h:1103-1121) instead of wrapping; fields stay periodic */
+#define IPIC_FLAG_DIPOLE_MOVER 32u /* the mover as shipped (cell.h:919-945): E = 0 and the analytic
+                                     field of a z dipole at the coordinate origin, strength
+                                     ipic_desc.dipole_moment; the field grids are not gathered */
 #define IPIC_FLAG_UNIFORM_BACKGROUND 16u /* upload_fields keeps only the uniform part of the
                                      fields as static background; the rest becomes the evolving
                                      FP32 state (needed by the field operator) */
@@ -61,13 +64,16 @@ typedef struct ipic_desc {
     double dt;                    /* UniverseProperties::dt */
     double c;                     /* UniverseProperties::speedOfLight */
     ipic_species_desc species[IPIC_MAX_SPECIES];
-    int32_t field_substeps;       /* leapfrog substeps per particle step with J held fixed (>= 1) */
+    int32_t field_substeps;       /* leapfrog substeps per particle step with J held fixed; 0: static
+                                     field (solveFieldStatically, field.h:272-284) */
     int32_t sink;                 /* 1: remove particles with |x - sink_center|^2 <= sink_radius^2
                                      (cell.h:1123-1128, planetRadius / objectCenter) */
     double sink_center[3];
     double sink_radius;
     int32_t tile[3];              /* cells per tile along each axis; 0 = automatic (<= 8, divides n) */
     uint32_t flags;               /* IPIC_FLAG_* */
+    double dipole_moment;         /* -externalMagneticField.z * planetRadius^3 (cell.h:921), read
+                                     with IPIC_FLAG_DIPOLE_MOVER */
 } ipic_desc;
 
 /* DurationMeasurement (simulator.h:37-40) plus counters of the call. */

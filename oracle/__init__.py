@@ -29,7 +29,8 @@ class Props(C.Structure):
         ("sink_center", C.c_double * 3),
         ("sink_radius", C.c_double),
         ("bc", C.c_int32),
-        ("pad_", C.c_int32),
+        ("mover", C.c_int32),
+        ("dipole_moment", C.c_double),
     ]
 
 
@@ -108,7 +109,8 @@ class Universe:
     """N-periodic FP64 universe of the oracle (cells, E/Bc/Bn/J grids)."""
 
     def __init__(self, n, w=(1.0, 1.0, 1.0), origin=(0.0, 0.0, 0.0), dt=0.1, c=1.0,
-                 field_substeps=1, sink_center=None, sink_radius=0.0, bc="periodic"):
+                 field_substeps=1, sink_center=None, sink_radius=0.0, bc="periodic", mover="gather",
+                 dipole_moment=0.0):
         if np.isscalar(n):
             n = (n, n, n)
         p = Props()
@@ -121,6 +123,8 @@ class Universe:
             p.sink_center[:] = [float(v) for v in sink_center]
             p.sink_radius = float(sink_radius)
         p.bc = {"periodic": 0, "reflect": 1}[bc]
+        p.mover = {"gather": 0, "dipole": 1}[mover]
+        p.dipole_moment = float(dipole_moment)
         self.props = p
         self.n = tuple(int(v) for v in n)
         self.ncell = self.n[0] * self.n[1] * self.n[2]

@@ -244,7 +244,7 @@ or_universe* or_create(const or_props* props) {
     if (!props || props->n[0] <= 0 || props->n[1] <= 0 || props->n[2] <= 0) return NULL;
     or_universe* u = (or_universe*)calloc(1, sizeof(or_universe));
     u->p = *props;
-    if (u->p.field_substeps < 1) u->p.field_substeps = 1;
+    if (u->p.field_substeps < 0) u->p.field_substeps = 0;
     u->ncell = props->n[0] * props->n[1] * props->n[2];
     u->count = (int64_t*)calloc((size_t)u->ncell, sizeof(int64_t));
     u->cap = (int64_t*)calloc((size_t)u->ncell, sizeof(int64_t));
@@ -387,8 +387,8 @@ void or_field_solve(or_universe* u, int nthreads) {
     const int64_t* n = P->n;
     const double* w = P->w;
     const double c2 = P->c * P->c;
-    const int ks = P->field_substeps;
-    const double dt = P->dt / (double)ks;
+    const int ks = P->field_substeps;  /* 0: static field (field.h:272-284) */
+    const double dt = ks > 0 ? P->dt / (double)ks : 0.0;
     set_threads(nthreads);
     for (int s = 0; s < ks; ++s) {
 #pragma omp parallel for schedule(static)
@@ -431,9 +431,11 @@ void or_field_solve(or_universe* u, int nthreads) {
     }
 }
 
-/* cell.h:896-964 with the field gather re-enabled (cell.h:905-917, 932-937): corners are
- * the 8 nodes of the particle's cell, interpolated with trilinearInterpolationF2P and the
- * cell volume; adaptive sub-cycling per cell.h:948-952 (|B|^2 as written there). */
+/* cell.h:896-964.  mover 0: the field gather re-enabled (cell.h:905-917, 932-937): corners
+ * are the 8 nodes of the particle's cell, interpolated with trilinearInterpolationF2P and the
+ * cell volume.  mover 1: the mover as shipped, an analytic dipole B at the particle's absolute
+ * position with E = 0 (cell.h:919-945).  Adaptive sub-cycling per cell.h:948-952 (|B|^2 as
+ * written there), B held over the sub-cycles. */
 void or_move(or_universe* u, int nthreads) {
     const or_props* P = &u->p;
     const int64_t* n = P->n;
@@ -458,10 +460,20 @@ void or_move(or_universe* u, int nthreads) {
         for (int64_t q = 0; q < u->count[c]; ++q) {
             or_particle* p = &u->cell[c][q];
             double rel[3], E[3], B[3];
-            for (int d = 0; d < 3; ++d) rel[d] = (p->pos[d] - org[d]) / P->w[d];
-            for (int d = 0; d < 3; ++d) {
-                E[d] = or_trilinear(Es[d], rel, vol);
-                B[d] = or_trilinear(Bs[d], rel, vol);
+            if (P->mover == 1) {
+                /* cell.h:919-945: dipole at the coordinate origin, E = 0 */
+                const double* x = p->pos;
+                const double fac1 = P->dipole_moment / pow(x[0] * x[0] + x[1] * x[1] + x[2] * x[2], 2.5);
+                E[0] = E[1] = E[2] = 0.0;
+                B[0] = 3.0 * x[0] * x[2] * fac1;
+                B[1] = 3.0 * x[1] * x[2] * fac1;
+                B[2] = (2.0 * pow(x[2], 2) - pow(x[0], 2) - pow(x[1], 2)) * fac1;
+            } else {
+                for (int d = 0; d < 3; ++d) rel[d] = (p->pos[d] - org[d]) / P->w[d];
+                for (int d = 0; d < 3; ++d) {
+                    E[d] = or_trilinear(Es[d], rel, vol);
+                    B[d] = or_trilinear(Bs[d], rel, vol);
+                }
             }
             const double B_mag = B[0] * B[0] + B[1] * B[1] + B[2] * B[2];
             double dt_sub = M_PI * P->c / (4.0 * fabs(p->qom) * B_mag);

@@ -43,13 +43,18 @@ typedef struct {
     double origin[3];        /* physical origin of cell (0,0,0) */
     double dt;
     double c;                /* speed of light */
-    int32_t field_substeps;  /* leapfrog substeps per particle step (J held fixed), >= 1 */
+    int32_t field_substeps;  /* leapfrog substeps per particle step (J held fixed); 0: the static
+                                field of solveFieldStatically (field.h:272-284) */
     int32_t sink;            /* 1: remove particles with |x - sink_center|^2 <= r^2 */
     double sink_center[3];
     double sink_radius;
     int32_t bc;              /* 0: periodic (D1); 1: the reference's reflection at the global
                                 faces, cell.h:1103-1121 (shift by half a cell, negate v) */
-    int32_t pad_;
+    int32_t mover;           /* 0: trilinear gather of E, node B (cell.h:905-917, 932-937);
+                                1: the reference's mover as shipped: analytic dipole B at the
+                                absolute position, E = 0 (cell.h:919-945) */
+    double dipole_moment;    /* magneticFieldTemp = -externalMagneticField.z * planetRadius^3
+                                (cell.h:921), used by mover 1 */
 } or_props;
 
 typedef struct or_universe or_universe;

@@ -349,3 +349,42 @@ def test_reflecting_faces_match_oracle():
     _compare_fields(u, sim)
     sim.close()
     u.close()
+
+
+def test_shipped_default_policies_dipole_mover_reflect_sink_static_field():
+    """The reference's default simulateSteps composition (simulator.h:244-276): static field
+    (field.h:272-284), the mover as shipped (analytic dipole B at the absolute position, E = 0,
+    sub-cycled, cell.h:896-964) and the export with reflection at the global faces and the
+    planet sink (cell.h:1103-1128), against the oracle on a box centred on the planet."""
+    import allscale_ipic3d_amd as G
+    n = (8, 8, 8)
+    origin = (-4.0, -4.0, -4.0)
+    R, B1z, dt, qom = 1.0, 0.5, 0.15, -25.0
+    rng = np.random.default_rng(31)
+    npart = 4000
+    pos = rng.uniform(-4.0, 4.0, size=(npart, 3))
+    pos = pos[np.sum(pos * pos, axis=1) > (1.2 * R) ** 2]
+    vel = rng.normal(0.0, 0.2, size=(len(pos), 3)) + np.array([0.05, 0.0, 0.0])
+    q = -0.01
+    u = O.Universe(n, origin=origin, dt=dt, field_substeps=0, sink_center=(0.0, 0.0, 0.0), sink_radius=R,
+                   bc="reflect", mover="dipole", dipole_moment=-B1z * R ** 3)
+    u.add_particles(O.make_particles(pos, vel, q, qom))
+    sim = G.Simulation(n, origin=origin, dt=dt, species=((q, qom),), field_substeps=0,
+                       sink_center=(0.0, 0.0, 0.0), sink_radius=R, flags=G.FLAG_REFLECT,
+                       dipole_moment=-B1z * R ** 3)
+    sim.upload_particles(0, pos, vel)
+    u.step(5)
+    st = sim.step(5)
+    ref, _ = u.particles()
+    got = sim.download_particles(0)
+    print("particles", len(ref), "removed by the sink", st["removed"])
+    assert st["removed"] > 0
+    assert len(got["pos"]) == len(ref)
+    idx, _ = match(ref["pos"], ref["vel"], got["pos"], got["vel"], n, 0.2)
+    assert len(np.unique(idx)) == len(idx)
+    dpos = got["pos"] - ref["pos"][idx]
+    assert float(np.max(np.abs(dpos))) < TOL
+    assert rel_err(got["vel"], ref["vel"][idx]) < TOL
+    np.testing.assert_array_equal(sim.cell_counts(), u.cell_counts())
+    sim.close()
+    u.close()

@@ -522,3 +522,51 @@ def test_full_step_conserves_count_and_charge():
     u.step(5)
     assert u.num_particles() == 2 * 8 * 64
     assert u.total_charge() == pytest.approx(q0, abs=1e-15)
+
+
+# ------------------------------------------------------- mover as shipped (dipole) -------
+
+def test_dipole_mover_field_is_the_initial_dipole_field():
+    """The shipped mover's B (cell.h:919-945) is the Bext of getDipoleFieldAt (field.h:43-78,
+    pinned by the field_test KATs above) for a planet at the origin, outside its radius; with
+    E = 0 one oracle move equals sub-cycled Boris steps (the Boris KAT pins those) in it."""
+    R, B1z, dt = 0.5, 20.0, 0.15
+    pos = np.array([1.1, -0.7, 0.9])
+    vel = np.array([0.3, 0.2, -0.4])
+    E, Bext, B = np.zeros(3), np.zeros(3), np.zeros(3)
+    O.lib().or_dipole_field(O._p(O.f64(pos)), O._p(O.f64([0, 0, 0])), O._p(O.f64([0, 0, 0])),
+                            O._p(O.f64([0, 0, 0])), R, B1z, O._p(E), O._p(Bext), O._p(B))
+    qom = -25.0
+    n = (4, 4, 4)
+    u = O.Universe(n, w=(1.0, 1.0, 1.0), origin=(-2.0, -2.0, -2.0), dt=dt, mover="dipole",
+                   dipole_moment=-B1z * R ** 3)
+    u.add_particles(O.make_particles(pos, vel, -1.0, qom))
+    u.move()
+    got, _ = u.particles()
+    u.close()
+    b2 = float(np.dot(Bext, Bext))
+    dt_sub = np.pi * 1.0 / (4.0 * abs(qom) * b2)
+    sub = min(int(dt / dt_sub) + 1, 100)
+    assert sub > 1  # the sub-cycling path is exercised
+    p = O.make_particles(pos, vel, -1.0, qom)
+    for _ in range(sub):
+        O.lib().or_update_velocity(O._p(p), O._p(O.f64([0, 0, 0])), O._p(O.f64(Bext)), dt / sub)
+        O.lib().or_update_position(O._p(p), dt / sub)
+    np.testing.assert_allclose(got["pos"][0], p["pos"][0], rtol=0, atol=1e-14)
+    np.testing.assert_allclose(got["vel"][0], p["vel"][0], rtol=0, atol=1e-14)
+
+
+def test_static_field_policy_leaves_fields():
+    """field_substeps = 0 is solveFieldStatically (field.h:272-284): E and Bc stay as set."""
+    n = (4, 4, 4)
+    rng = np.random.default_rng(5)
+    E = rng.normal(size=(64, 3))
+    Bc = rng.normal(size=(64, 3))
+    u = O.Universe(n, dt=0.1, field_substeps=0)
+    u.set_fields(E, Bc)
+    u.set_J(rng.normal(size=(64, 3)))
+    u.field_solve()
+    f = u.fields()
+    u.close()
+    np.testing.assert_array_equal(f["E"], E)
+    np.testing.assert_array_equal(f["Bc"], Bc)
