@@ -1,7 +1,9 @@
 // Drives the C++ mirror (include/ipic3d_gpu/ipic3d.hpp) the way the reference's simulation
 // tests drive theirs: build a Universe, place particles, set fields, simulateSteps.
 //
-//   ipic3d_mirror_test <input.bin> <output.bin>   run the case of input.bin on the GPU
+//   ipic3d_mirror_test <input.bin> <output.bin>   run the case of input.bin on the GPU; also
+//                                                 writes <output.bin>.counts.out and
+//                                                 <output.bin>.log in the reference's formats
 //   ipic3d_mirror_test --expect-no-device          check the error path without a GPU
 //
 // input.bin (little-endian): int64 n, int64 steps, int64 nparticles, double dt, then per
@@ -13,7 +15,7 @@
 #include <iostream>
 #include <set>
 
-#include "ipic3d_gpu/ipic3d.hpp"
+#include "ipic3d_gpu/io.hpp"
 
 using namespace ipic3d_gpu;
 
@@ -70,7 +72,12 @@ int main(int argc, char** argv) {
         } catch (const Error& e) {
             guard = e.code() == IPIC_E_STATE ? 1 : 0;
         }
+        std::ofstream log(std::string(argv[2]) + ".log");
+        writeOutputHeader(log);
+        writeOutputData(0, (int)steps, u, log, std::string(argv[2]) + ".density", 1, 0);
         const DurationMeasurement d = simulateSteps(steps, u);
+        writeOutputData((int)steps - 1, (int)steps, u, log, std::string(argv[2]) + ".density", 1, 0);
+        outputNumberOfParticlesPerCell(u, std::string(argv[2]) + ".counts.out");
         const int64_t count = u.countParticlesInDomain();
         const std::vector<int64_t> cells = u.getParticleCountsPerCell();
         const double ke = u.conservation().kinetic_energy;

@@ -88,4 +88,11 @@ def test_cpp_mirror_matches_python_binding(tmp_path):
     np.testing.assert_array_equal(cells, sim.cell_counts())
     ke_py = sim.conservation()["kinetic_energy"]
     assert abs(ke - ke_py) <= 1e-12 * abs(ke_py)
+    # the reference's output formats from C++ (io.hpp) and Python (io.py) agree
+    from allscale_ipic3d_amd import io as IO
+    IO.write_cell_counts(tmp_path / "py.counts.out", sim.cell_counts(), (n, n, n))
+    assert (tmp_path / "out.bin.counts.out").read_bytes() == (tmp_path / "py.counts.out").read_bytes()
+    rows = (tmp_path / "out.bin.log").read_text().splitlines()
+    assert rows[0].startswith("Cycle \t Total Moment") and len(rows) == 3
+    assert rows[2].split("\t")[0] == str(steps - 1)
     sim.close()
