@@ -20,6 +20,7 @@ FLAG_PHASE_TIMING = 4
 FLAG_REFLECT = 8
 FLAG_UNIFORM_BACKGROUND = 16
 FLAG_DIPOLE_MOVER = 32
+FLAG_FORWARD_SOLVER = 64
 
 
 class SpeciesDesc(C.Structure):

@@ -972,7 +972,9 @@ struct FieldArgs {
     int* jflag;
     const int* jpairs;  // [3][nmax][1+JPMAX]: count, then (tile << 8 | local)
     float dt;           // field substep
-    float c2;
+    float c2;           // c^2 (field energies)
+    float eF;           // factor of curl- B in the E pass: c^2 (leapfrog) or c (forward solver)
+    float bF;           // factor of curl+ E in the B pass: 1 (leapfrog) or c (forward solver)
 };
 
 __device__ __forceinline__ void xyz_of(const Geo& g, int i, int& x, int& y, int& z) {
@@ -1054,7 +1056,7 @@ __global__ __launch_bounds__(256) void k_field_E(FieldArgs F, int from_partials)
     const float4 bmy = F.Bc[nid_of(g, x, wrap1(y - 1, g.n[1]), z)];
     const float4 bmz = F.Bc[nid_of(g, x, y, wrap1(z - 1, g.n[2]))];
     float4 e = F.E[i];
-    const float dt = F.dt, c2 = F.c2;
+    const float dt = F.dt, c2 = F.eF;
     e.x = e.x + dt * (c2 * ((b.z - bmy.z) * g.inv_w[1] - (b.y - bmz.y) * g.inv_w[2]) - Jx);
     e.y = e.y + dt * (c2 * ((b.x - bmz.x) * g.inv_w[2] - (b.z - bmx.z) * g.inv_w[0]) - Jy);
     e.z = e.z + dt * (c2 * ((b.y - bmx.y) * g.inv_w[0] - (b.x - bmy.x) * g.inv_w[1]) - Jz);
@@ -1082,13 +1084,13 @@ __global__ __launch_bounds__(256) void k_field_B(FieldArgs F) {
     const float4 epy = F.E[nid_of(g, x, wrap1(y + 1, g.n[1]), z)];
     const float4 epz = F.E[nid_of(g, x, y, wrap1(z + 1, g.n[2]))];
     float4 b = F.Bc[i];
-    const float dt = F.dt;
+    const float dt = F.dt * F.bF;
     b.x = b.x - dt * ((epy.z - e.z) * g.inv_w[1] - (epz.y - e.y) * g.inv_w[2]);
     b.y = b.y - dt * ((epz.x - e.x) * g.inv_w[2] - (epx.z - e.z) * g.inv_w[0]);
     b.z = b.z - dt * ((epx.y - e.y) * g.inv_w[0] - (epy.x - e.x) * g.inv_w[1]);
-    if (F.SB) {
+    if (F.SB) {  // the source already carries the B-pass factor
         const float4 sb = F.SB[i];
-        b.x -= dt * sb.x; b.y -= dt * sb.y; b.z -= dt * sb.z;
+        b.x -= F.dt * sb.x; b.y -= F.dt * sb.y; b.z -= F.dt * sb.z;
     }
     F.Bc[i] = b;
 }
@@ -1626,6 +1628,9 @@ FieldArgs field_args(ipic_ctx* ctx) {
     F.jpairs = ctx->jpairs;
     F.dt = ctx->d.field_substeps > 0 ? (float)(ctx->d.dt / ctx->d.field_substeps) : 0.0f;
     F.c2 = (float)(ctx->d.c * ctx->d.c);
+    const bool fwd = (ctx->d.flags & IPIC_FLAG_FORWARD_SOLVER) != 0;
+    F.eF = fwd ? (float)ctx->d.c : F.c2;
+    F.bF = fwd ? (float)ctx->d.c : 1.0f;
     return F;
 }
 
@@ -2340,7 +2345,10 @@ int ipic_gpu_upload_fields(ipic_ctx* ctx, const double* E, const double* Bc) {
     std::vector<double> sE(n3), sB(n3), rBn(n3);
     const int* n = g.n;
     const double* w = ctx->d.cell_width;
-    const double c2 = ctx->d.c * ctx->d.c;
+    // curl factors of the field policy: leapfrog c^2 and 1, forward solver c and c
+    const bool fwd = (ctx->d.flags & IPIC_FLAG_FORWARD_SOLVER) != 0;
+    const double c2 = fwd ? ctx->d.c : ctx->d.c * ctx->d.c;
+    const double bF = fwd ? ctx->d.c : 1.0;
     auto id = [&](int x, int y, int z) {
         x = (x % n[0] + n[0]) % n[0];
         y = (y % n[1] + n[1]) % n[1];
@@ -2358,9 +2366,9 @@ int ipic_gpu_upload_fields(ipic_ctx* ctx, const double* E, const double* Bc) {
                 sE[3 * i + 0] = c2 * ((bB[3 * i + 2] - bB[3 * my + 2]) / w[1] - (bB[3 * i + 1] - bB[3 * mz + 1]) / w[2]);
                 sE[3 * i + 1] = c2 * ((bB[3 * i + 0] - bB[3 * mz + 0]) / w[2] - (bB[3 * i + 2] - bB[3 * mx + 2]) / w[0]);
                 sE[3 * i + 2] = c2 * ((bB[3 * i + 1] - bB[3 * mx + 1]) / w[0] - (bB[3 * i + 0] - bB[3 * my + 0]) / w[1]);
-                sB[3 * i + 0] = (bE[3 * py + 2] - bE[3 * i + 2]) / w[1] - (bE[3 * pz + 1] - bE[3 * i + 1]) / w[2];
-                sB[3 * i + 1] = (bE[3 * pz + 0] - bE[3 * i + 0]) / w[2] - (bE[3 * px + 2] - bE[3 * i + 2]) / w[0];
-                sB[3 * i + 2] = (bE[3 * px + 1] - bE[3 * i + 1]) / w[0] - (bE[3 * py + 0] - bE[3 * i + 0]) / w[1];
+                sB[3 * i + 0] = bF * ((bE[3 * py + 2] - bE[3 * i + 2]) / w[1] - (bE[3 * pz + 1] - bE[3 * i + 1]) / w[2]);
+                sB[3 * i + 1] = bF * ((bE[3 * pz + 0] - bE[3 * i + 0]) / w[2] - (bE[3 * px + 2] - bE[3 * i + 2]) / w[0]);
+                sB[3 * i + 2] = bF * ((bE[3 * px + 1] - bE[3 * i + 1]) / w[0] - (bE[3 * py + 0] - bE[3 * i + 0]) / w[1]);
                 for (int k = 0; k < 3; ++k) {
                     hasSE |= sE[3 * i + k] != 0.0;
                     hasSB |= sB[3 * i + k] != 0.0;
@@ -2484,6 +2492,8 @@ int ipic_gpu_set_field_operator(ipic_ctx* ctx, const ipic_field_operator* op) {
         return 0;
     }
     if (!op->params) return fail(ctx, IPIC_E_ARG, "null operator parameters");
+    if (ctx->d.flags & IPIC_FLAG_FORWARD_SOLVER)
+        return fail(ctx, IPIC_E_STATE, "the field operator stands in for leapfrog substeps");
     if (op->substeps < 1 || op->substeps != ctx->d.field_substeps)
         return fail(ctx, IPIC_E_ARG, "operator substeps differ from the context's field_substeps");
     if (!(op->in_E > 0) || !(op->in_B > 0) || !(op->in_J > 0) || !std::isfinite(op->out_E) || !std::isfinite(op->out_P))

@@ -44,6 +44,8 @@ extern "C" {
 #define IPIC_FLAG_DIPOLE_MOVER 32u /* the mover as shipped (cell.h:919-945): E = 0 and the analytic
                                      field of a z dipole at the coordinate origin, strength
                                      ipic_desc.dipole_moment; the field grids are not gathered */
+#define IPIC_FLAG_FORWARD_SOLVER 64u /* field policy: the explicit forward solver (solveFieldForward,
+                                     field.h:293-337) instead of leapfrog */
 #define IPIC_FLAG_UNIFORM_BACKGROUND 16u /* upload_fields keeps only the uniform part of the
                                      fields as static background; the rest becomes the evolving
                                      FP32 state (needed by the field operator) */

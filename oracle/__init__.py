@@ -31,6 +31,8 @@ class Props(C.Structure):
         ("bc", C.c_int32),
         ("mover", C.c_int32),
         ("dipole_moment", C.c_double),
+        ("solver", C.c_int32),
+        ("pad_", C.c_int32),
     ]
 
 
@@ -110,7 +112,7 @@ class Universe:
 
     def __init__(self, n, w=(1.0, 1.0, 1.0), origin=(0.0, 0.0, 0.0), dt=0.1, c=1.0,
                  field_substeps=1, sink_center=None, sink_radius=0.0, bc="periodic", mover="gather",
-                 dipole_moment=0.0):
+                 dipole_moment=0.0, solver="leapfrog"):
         if np.isscalar(n):
             n = (n, n, n)
         p = Props()
@@ -125,6 +127,7 @@ class Universe:
         p.bc = {"periodic": 0, "reflect": 1}[bc]
         p.mover = {"gather": 0, "dipole": 1}[mover]
         p.dipole_moment = float(dipole_moment)
+        p.solver = {"leapfrog": 0, "forward": 1}[solver]
         self.props = p
         self.n = tuple(int(v) for v in n)
         self.ncell = self.n[0] * self.n[1] * self.n[2]

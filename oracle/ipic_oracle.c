@@ -379,7 +379,8 @@ void or_deposit(or_universe* u, int nthreads) {
     }
 }
 
-/* field.h:356-372 as two passes over the N-periodic grid (D2, D3), repeated
+/* field.h:356-372 (solver 0) or field.h:293-337 (solver 1, the explicit forward solver) as
+ * two passes over the N-periodic grid (D2, D3), repeated
  * field_substeps times with dt/field_substeps and J held fixed, then interpC2N
  * (field.h:234-247) to refresh node B for the gather. */
 void or_field_solve(or_universe* u, int nthreads) {
@@ -389,6 +390,10 @@ void or_field_solve(or_universe* u, int nthreads) {
     const double c2 = P->c * P->c;
     const int ks = P->field_substeps;  /* 0: static field (field.h:272-284) */
     const double dt = ks > 0 ? P->dt / (double)ks : 0.0;
+    /* leapfrog: c^2 curl- B in the E pass, 1 x curl+ E in the B pass; forward solver
+     * (field.h:308, 322): c curl B and c curl E */
+    const double eF = P->solver == 1 ? P->c : c2;
+    const double bF = P->solver == 1 ? P->c : 1.0;
     set_threads(nthreads);
     for (int s = 0; s < ks; ++s) {
 #pragma omp parallel for schedule(static)
@@ -400,9 +405,9 @@ void or_field_solve(or_universe* u, int nthreads) {
             const double* bmz = u->Bc + 3 * pidx(n, x, y, z - 1);
             const double* j = u->J + 3 * i;
             double* e = u->E + 3 * i;
-            e[0] = e[0] + dt * (c2 * ((b[2] - bmy[2]) / w[1] - (b[1] - bmz[1]) / w[2]) - j[0]);
-            e[1] = e[1] + dt * (c2 * ((b[0] - bmz[0]) / w[2] - (b[2] - bmx[2]) / w[0]) - j[1]);
-            e[2] = e[2] + dt * (c2 * ((b[1] - bmx[1]) / w[0] - (b[0] - bmy[0]) / w[1]) - j[2]);
+            e[0] = e[0] + dt * (eF * ((b[2] - bmy[2]) / w[1] - (b[1] - bmz[1]) / w[2]) - j[0]);
+            e[1] = e[1] + dt * (eF * ((b[0] - bmz[0]) / w[2] - (b[2] - bmx[2]) / w[0]) - j[1]);
+            e[2] = e[2] + dt * (eF * ((b[1] - bmx[1]) / w[0] - (b[0] - bmy[0]) / w[1]) - j[2]);
         }
 #pragma omp parallel for schedule(static)
         for (int64_t i = 0; i < u->ncell; ++i) {
@@ -412,9 +417,9 @@ void or_field_solve(or_universe* u, int nthreads) {
             const double* epy = u->E + 3 * pidx(n, x, y + 1, z);
             const double* epz = u->E + 3 * pidx(n, x, y, z + 1);
             double* b = u->Bc + 3 * i;
-            b[0] = b[0] - dt * ((epy[2] - e[2]) / w[1] - (epz[1] - e[1]) / w[2]);
-            b[1] = b[1] - dt * ((epz[0] - e[0]) / w[2] - (epx[2] - e[2]) / w[0]);
-            b[2] = b[2] - dt * ((epx[1] - e[1]) / w[0] - (epy[0] - e[0]) / w[1]);
+            b[0] = b[0] - dt * bF * ((epy[2] - e[2]) / w[1] - (epz[1] - e[1]) / w[2]);
+            b[1] = b[1] - dt * bF * ((epz[0] - e[0]) / w[2] - (epx[2] - e[2]) / w[0]);
+            b[2] = b[2] - dt * bF * ((epx[1] - e[1]) / w[0] - (epy[0] - e[0]) / w[1]);
         }
     }
 #pragma omp parallel for schedule(static)

@@ -55,6 +55,9 @@ typedef struct {
                                 absolute position, E = 0 (cell.h:919-945) */
     double dipole_moment;    /* magneticFieldTemp = -externalMagneticField.z * planetRadius^3
                                 (cell.h:921), used by mover 1 */
+    int32_t solver;          /* 0: leapfrog (field.h:356-372); 1: the explicit forward solver
+                                (field.h:293-337): E += dt (c curl B - J), Bc -= c dt curl E */
+    int32_t pad_;
 } or_props;
 
 typedef struct or_universe or_universe;

@@ -136,6 +136,31 @@ def test_fields_only_leapfrog():
     assert cu["divB_l2"] == pytest.approx(l2, rel=1e-4)
 
 
+@pytest.mark.parametrize("substeps", [0, 2])
+def test_fields_only_forward_and_static_solvers(substeps):
+    """The explicit forward solver (field.h:293-337, c != 1 so its single c factors matter)
+    and the static field (field.h:272-284, substeps 0) against the oracle."""
+    import allscale_ipic3d_amd as G
+    rng = np.random.default_rng(6)
+    n = (6, 8, 10)
+    w, c = (0.5, 1.0, 2.0), 1.7
+    u = O.Universe(n, w=w, dt=0.02, c=c, field_substeps=substeps, solver="forward")
+    ncell = int(np.prod(n))
+    E = rng.normal(size=(ncell, 3))
+    Bc = rng.normal(size=(ncell, 3))
+    u.set_fields(E, Bc)
+    sim = G.Simulation(n, cell_width=w, dt=0.02, c=c, species=((-1.0, -1.0),), field_substeps=substeps,
+                       flags=G.FLAG_FORWARD_SOLVER)
+    sim.upload_fields(E, Bc)
+    for _ in range(10):
+        u.deposit(); u.field_solve()
+    sim.step(10)
+    _compare_fields(u, sim, names=("E", "Bc", "Bn"))
+    if substeps == 0:
+        f = sim.download_fields()
+        np.testing.assert_allclose(f["E"], E, rtol=0, atol=1e-6)
+
+
 @pytest.mark.parametrize("n", [(1, 1, 1), (2, 2, 2), (3, 3, 3), (2, 3, 5), (9, 4, 6)])
 def test_tiny_and_ragged_grids_fast_particles(n):
     # few particles with speeds up to 1.7 cells per step: multi-cell moves, periodic
@@ -388,3 +413,31 @@ def test_shipped_default_policies_dipole_mover_reflect_sink_static_field():
     np.testing.assert_array_equal(sim.cell_counts(), u.cell_counts())
     sim.close()
     u.close()
+
+
+def test_tracer_frames_match_oracle():
+    """pic_tracer.cxx:77-148 on the GPU (allscale_ipic3d_amd.tracer) against the oracle
+    composed the same way (dipole mover, static field, periodic wrap, planet sink)."""
+    from allscale_ipic3d_amd import tracer
+    n, w, origin = (8, 8, 8), (1.0, 1.0, 1.0), (-4.0, -4.0, -4.0)
+    R, B1z, dt, q, qom = 1.0, 0.4, 0.1, 1.0, 1.0
+    rng = np.random.default_rng(41)
+    rel = rng.uniform(-3.5, 3.5, size=(3000, 3))
+    vel = rng.normal(0.0, 0.3, size=(3000, 3))
+    frames, started = tracer.trace(rel, vel, q, qom, n, w, origin, dt, R, (0.0, 0.0, 0.0), B1z, steps=12,
+                                   frame_interval=4)
+    keep = np.sum(rel * rel, axis=1) >= R * R
+    u = O.Universe(n, w=w, origin=origin, dt=dt, field_substeps=0, sink_center=(0.0, 0.0, 0.0), sink_radius=R,
+                   mover="dipole", dipole_moment=-B1z * R ** 3)
+    u.add_particles(O.make_particles(rel[keep], vel[keep], q, qom))
+    ref = []
+    for t in range(12):
+        if t % 4 == 0:
+            ref.append(u.cell_counts())
+        u.step(1)
+    u.close()
+    assert started == int(keep.sum())
+    assert frames.shape == (3, 512)
+    for a, b in zip(frames, ref):
+        assert a.sum() == b.sum()
+        assert int(np.abs(a - b).sum()) <= 4  # FP32 / FP64 face ties

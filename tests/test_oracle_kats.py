@@ -570,3 +570,47 @@ def test_static_field_policy_leaves_fields():
     u.close()
     np.testing.assert_array_equal(f["E"], E)
     np.testing.assert_array_equal(f["Bc"], Bc)
+
+
+# -------------------------------------------------------------- forward solver -----------
+
+def test_forward_solver_kat():
+    """field_test.cc:418-462 (solveFieldForward): uniform E 0.2, Bc 0.8, J 0.6, dt 1 ->
+    E = -0.4, Bc unchanged."""
+    n = (5, 5, 5)
+    u = O.Universe(n, dt=1.0, solver="forward")
+    nc = 125
+    u.set_fields(np.full((nc, 3), 0.2), np.full((nc, 3), 0.8))
+    u.set_J(np.full((nc, 3), 0.6))
+    u.field_solve()
+    f = u.fields()
+    u.close()
+    np.testing.assert_allclose(f["E"], -0.4, rtol=0, atol=1e-15)
+    np.testing.assert_allclose(f["Bc"], 0.8, rtol=0, atol=1e-15)
+
+
+def test_forward_solver_speed_of_light_factors():
+    """The forward solver's curls carry one factor c each (field.h:308, 322): a periodic linear
+    ramp Bc_z = a*y (a jump at the wrap) gives dE_x = c dt a / w_y at interior nodes, and
+    E_y = b*z gives dBc_x = +c dt b / w_z; leapfrog gives c^2 and 1 instead."""
+    n, w, dt, c, a, b = (4, 6, 6), (1.0, 0.5, 2.0), 0.01, 3.0, 0.7, 0.3
+    x, y, z = np.meshgrid(*[np.arange(k) for k in n], indexing="ij")
+    nc = np.prod(n)
+    for solver, eF, bF in (("forward", c, c), ("leapfrog", c * c, 1.0)):
+        E = np.zeros((nc, 3))
+        Bc = np.zeros((nc, 3))
+        Bc[:, 2] = (a * y).reshape(-1)
+        E[:, 1] = (b * z).reshape(-1)
+        u = O.Universe(n, w=w, dt=dt, c=c, solver=solver)
+        u.set_fields(E, Bc)
+        u.set_J(np.zeros((nc, 3)))
+        u.field_solve()
+        f = u.fields()
+        u.close()
+        inner_y = (y >= 1).reshape(-1)
+        dEx = f["E"][:, 0] - E[:, 0]
+        np.testing.assert_allclose(dEx[inner_y], eF * dt * a / w[1], rtol=1e-12)
+        # curl+ E _x = -(dEy/dz) with E updated first (E_y unchanged by the ramp in Bc_z)
+        inner_z = (z <= n[2] - 2).reshape(-1)
+        dBx = f["Bc"][:, 0] - Bc[:, 0]
+        np.testing.assert_allclose(dBx[inner_z], bF * dt * b / w[2], rtol=1e-12)
