@@ -108,13 +108,35 @@ class FieldState(C.Structure):
     ]
 
 
+class RefDistribution(C.Structure):
+    """ipic_ref_distribution: distribution::uniform_pos_normal_speed (cell.h:405-440)."""
+    _fields_ = [
+        ("pos_min", C.c_double * 3), ("pos_max", C.c_double * 3),
+        ("vel_mean", C.c_double * 3), ("vel_stddev", C.c_double * 3),
+        ("num_particles", C.c_uint64), ("seed", C.c_uint32), ("nthreads", C.c_int32),
+    ]
+
+
+def ref_distribution(pos_min, pos_max, vel_mean, vel_stddev, num_particles, seed=0, nthreads=0):
+    d = RefDistribution()
+    d.pos_min[:] = [float(v) for v in pos_min]
+    d.pos_max[:] = [float(v) for v in pos_max]
+    d.vel_mean[:] = [float(v) for v in vel_mean]
+    d.vel_stddev[:] = [float(v) for v in vel_stddev]
+    d.num_particles = int(num_particles)
+    d.seed = int(seed) & 0xFFFFFFFF
+    d.nthreads = int(nthreads)
+    return d
+
+
 # every symbol declared in include/ipic_gpu.h
 ABI_SYMBOLS = (
     "ipic_gpu_create", "ipic_gpu_destroy", "ipic_gpu_last_error", "ipic_gpu_upload_particles",
     "ipic_gpu_init_particles", "ipic_gpu_upload_fields", "ipic_gpu_step", "ipic_gpu_num_particles",
     "ipic_gpu_cell_counts", "ipic_gpu_download_particles", "ipic_gpu_download_fields",
     "ipic_gpu_conservation", "ipic_gpu_device_info", "ipic_gpu_field_step", "ipic_gpu_particle_step",
-    "ipic_gpu_set_field_operator", "ipic_gpu_field_state",
+    "ipic_gpu_set_field_operator", "ipic_gpu_field_state", "ipic_ref_cell_counts", "ipic_ref_particles",
+    "ipic_ref_last_error", "ipic_gpu_init_reference",
 )
 
 _lib = None
@@ -151,8 +173,13 @@ def load_library(path: str = LIB_PATH):
     L.ipic_gpu_particle_step.argtypes = [P, C.POINTER(StepStats)]
     L.ipic_gpu_set_field_operator.argtypes = [P, C.POINTER(FieldOperatorDesc)]
     L.ipic_gpu_field_state.argtypes = [P, C.POINTER(FieldState)]
+    L.ipic_ref_cell_counts.argtypes = [C.POINTER(Desc), C.POINTER(RefDistribution), P]
+    L.ipic_ref_particles.argtypes = [C.POINTER(Desc), C.POINTER(RefDistribution), P, P, P]
+    L.ipic_ref_last_error.restype = C.c_char_p
+    L.ipic_ref_last_error.argtypes = []
+    L.ipic_gpu_init_reference.argtypes = [P, I32, C.POINTER(RefDistribution), P]
     for name in ABI_SYMBOLS:
-        if name != "ipic_gpu_last_error":
+        if name not in ("ipic_gpu_last_error", "ipic_ref_last_error"):
             getattr(L, name).restype = C.c_int
     _lib = L
     return L
@@ -167,6 +194,25 @@ def _f64(a, shape=None):
     if shape is not None:
         a = a.reshape(shape)
     return a
+
+
+def reference_init(desc, dist, particles=True):
+    """The reference's initCells streams on the host (ipic_ref_cell_counts / ipic_ref_particles):
+    per-cell counts (row-major) and, with particles=True, positions and velocities (n x 3
+    doubles, row-major cells, draw order within a cell).  No GPU is used."""
+    L = load_library()
+    ncell = int(desc.n[0]) * int(desc.n[1]) * int(desc.n[2])
+    counts = np.zeros(ncell, dtype=np.int64)
+    if L.ipic_ref_cell_counts(C.byref(desc), C.byref(dist), _ptr(counts)) != 0:
+        raise IpicError(L.ipic_ref_last_error().decode())
+    if not particles:
+        return counts
+    n = int(counts.sum())
+    pos = np.zeros((n, 3), dtype=np.float64)
+    vel = np.zeros((n, 3), dtype=np.float64)
+    if L.ipic_ref_particles(C.byref(desc), C.byref(dist), _ptr(counts), _ptr(pos), _ptr(vel)) != 0:
+        raise IpicError(L.ipic_ref_last_error().decode())
+    return counts, pos, vel
 
 
 def device_info():
@@ -257,6 +303,13 @@ class Simulation:
             uni = 0
         self._check(self._L.ipic_gpu_init_particles(self._h, int(species), _ptr(arr), uni, _ptr(_f64(vth)),
                                                     _ptr(_f64(drift)), int(seed)))
+
+    def init_reference(self, species, dist):
+        """initCells with the reference's random streams (ipic_gpu_init_reference); returns the
+        phase-1 per-cell counts."""
+        counts = np.zeros(self.ncell, dtype=np.int64)
+        self._check(self._L.ipic_gpu_init_reference(self._h, int(species), C.byref(dist), _ptr(counts)))
+        return counts
 
     def upload_fields(self, E=None, Bc=None):
         e = None if E is None else _f64(E, (self.ncell, 3))

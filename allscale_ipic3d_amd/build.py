@@ -21,7 +21,7 @@ HIPFLAGS = [
 ]
 
 LIBS = {
-    "libipic_gpu.so": ["csrc/ipic_gpu.hip", "csrc/ipic_nop.hip"],
+    "libipic_gpu.so": ["csrc/ipic_gpu.hip", "csrc/ipic_nop.hip", "csrc/ipic_refinit.cpp"],
 }
 
 

@@ -78,7 +78,9 @@ struct Geo {
     float pi_c;
     int sink;
     int reflect;          // 1: reflect at the global faces (cell.h:1103-1121) instead of wrapping
-    float dip_m;          // -B1z R^3 of the mover as shipped (analytic dipole at the origin)
+    float dip_m;          // -B1z R^3 / L^3 of the mover as shipped (analytic dipole at the origin)
+    float dip_o[3];       // origin / L and cell width / L: the dipole is evaluated in units of
+    float dip_w[3];       // L = cell_width[0], as |x|^5 of a box in metres overflows FP32
     float sink_c[3];
     float sink_r2;
     float E0[3];          // uniform part of the background field (added in the gather)
@@ -423,12 +425,13 @@ __device__ __forceinline__ void reflect_faces(const Geo& g, int cx, int cy, int 
 
 // The mover as shipped (cell.h:919-945): E = 0 and the field of a z dipole at the coordinate
 // origin, B = m (3xz, 3yz, 2z^2 - x^2 - y^2) / |x|^5 with m = -B1z R^3, at the absolute
-// position of offsets (px,py,pz) in cell (cx,cy,cz).
+// position of offsets (px,py,pz) in cell (cx,cy,cz).  Lengths are in units of L (x' = x / L,
+// m' = m / L^3), which leaves B unchanged and keeps |x'|^5 in range.
 __device__ __forceinline__ void dipole_field(const Geo& g, int cx, int cy, int cz, float px, float py, float pz,
                                              float& Bx, float& By, float& Bz) {
-    const float x = g.origin[0] + ((float)cx + px) * g.w[0];
-    const float y = g.origin[1] + ((float)cy + py) * g.w[1];
-    const float z = g.origin[2] + ((float)cz + pz) * g.w[2];
+    const float x = g.dip_o[0] + ((float)cx + px) * g.dip_w[0];
+    const float y = g.dip_o[1] + ((float)cy + py) * g.dip_w[1];
+    const float z = g.dip_o[2] + ((float)cz + pz) * g.dip_w[2];
     const float r2 = x * x + y * y + z * z;
     const float fac = g.dip_m / (r2 * r2 * sqrtf(r2));
     Bx = 3.0f * x * z * fac;
@@ -2112,7 +2115,14 @@ int ipic_gpu_create(const ipic_desc* desc, ipic_ctx** out) {
     }
     g.sink = d.sink ? 1 : 0;
     g.reflect = (d.flags & IPIC_FLAG_REFLECT) ? 1 : 0;
-    g.dip_m = (float)d.dipole_moment;
+    {
+        const double L = d.cell_width[0];
+        g.dip_m = (float)(d.dipole_moment / (L * L * L));
+        for (int k = 0; k < 3; ++k) {
+            g.dip_o[k] = (float)(d.origin[k] / L);
+            g.dip_w[k] = (float)(d.cell_width[k] / L);
+        }
+    }
     g.sink_r2 = (float)(d.sink_radius * d.sink_radius);
 
     std::vector<int> jp;
@@ -2242,6 +2252,22 @@ int ipic_gpu_upload_particles(ipic_ctx* ctx, int32_t species, int64_t n, const d
     if (int rc = set_layout(ctx, species, start)) return rc;
     ctx->j_valid = false;
     return 0;
+}
+
+int ipic_gpu_init_reference(ipic_ctx* ctx, int32_t species, const ipic_ref_distribution* dist, int64_t* counts) {
+    if (int rc = check_ctx(ctx)) return rc;
+    if (species < 0 || species >= ctx->d.nspecies) return fail(ctx, IPIC_E_ARG, "species out of range");
+    const int64_t nc = (int64_t)ctx->d.n[0] * ctx->d.n[1] * ctx->d.n[2];
+    std::vector<int64_t> cnt((size_t)nc);
+    if (int rc = ipic_ref_cell_counts(&ctx->d, dist, cnt.data())) return fail(ctx, rc, ipic_ref_last_error());
+    int64_t total = 0;
+    for (int64_t c = 0; c < nc; ++c) total += cnt[(size_t)c];
+    if (total >= (1LL << 31)) return fail(ctx, IPIC_E_RANGE, "at most 2^31-1 particles per species");
+    std::vector<double> pos((size_t)total * 3), vel((size_t)total * 3);
+    if (int rc = ipic_ref_particles(&ctx->d, dist, cnt.data(), pos.data(), vel.data()))
+        return fail(ctx, rc, ipic_ref_last_error());
+    if (counts) std::memcpy(counts, cnt.data(), (size_t)nc * sizeof(int64_t));
+    return ipic_gpu_upload_particles(ctx, species, total, pos.data(), vel.data());
 }
 
 int ipic_gpu_init_particles(ipic_ctx* ctx, int32_t species, const int32_t* ppc, int32_t uniform_ppc,

@@ -213,6 +213,38 @@ typedef struct ipic_field_state {
 } ipic_field_state;
 int ipic_gpu_field_state(ipic_ctx* ctx, ipic_field_state* out);
 
+/* The reference's particle initialisation with its own random streams: initCells
+ * (cell.h:512-654) over distribution::uniform_pos_normal_speed (cell.h:405-440), as the main
+ * driver calls it through createUniverseFromDistribution (src/ipic3d.cxx:74-85,
+ * universe.h:85-103).  Positions are uniform in [pos_min, pos_max) per component
+ * (std::minstd_rand + std::uniform_real_distribution<double>, cell.h:138-175), velocities
+ * vel_mean + vel_stddev * the reference's ziggurat normal (ziggurat_normal_distribution.h,
+ * cell.h:223-257).  Phase 1 (cell.h:525-610) estimates the per-cell counts from 100 pseudo
+ * particles per cell of the distribution's own stream; phase 2 (cell.h:613-650) draws each
+ * cell's particles from streams seeded by the cell coordinate.  Host computation, bit-identical
+ * to the reference's streams; the geometry is desc's n, cell_width and origin.  Every
+ * particle gets the species the caller names (the reference sets q = e, qom = e / m_p). */
+typedef struct ipic_ref_distribution {
+    double pos_min[3], pos_max[3];     /* vector::uniform bounds of the positions */
+    double vel_mean[3], vel_stddev[3]; /* vector::normal of the velocities */
+    uint64_t num_particles;            /* total particles, initCells' numParticles */
+    uint32_t seed;                     /* the distribution's seed (positions seed+1, velocities seed+2) */
+    int32_t nthreads;                  /* host threads, 0: all */
+} ipic_ref_distribution;
+
+/* Phase 1: the number of particles of each cell (ncell int64, row-major).  Fails with
+ * IPIC_E_ARG where the reference's initCells asserts (no remainder to spread, cell.h:557). */
+int ipic_ref_cell_counts(const ipic_desc* desc, const ipic_ref_distribution* dist, int64_t* counts);
+/* Phase 2: the particles of all cells, row-major cells, each cell's particles in draw order;
+ * pos, vel: sum(counts) x 3 doubles (absolute positions). */
+int ipic_ref_particles(const ipic_desc* desc, const ipic_ref_distribution* dist, const int64_t* counts,
+                       double* pos, double* vel);
+/* Message of the last failed ipic_ref_* call on this thread. */
+const char* ipic_ref_last_error(void);
+/* Both phases, then ipic_gpu_upload_particles of the result into `species`.  counts (nullable)
+ * receives the phase-1 counts. */
+int ipic_gpu_init_reference(ipic_ctx* ctx, int32_t species, const ipic_ref_distribution* dist, int64_t* counts);
+
 /* Device properties used by the tests and benchmark. */
 int ipic_gpu_device_info(char* name, int32_t name_len, int32_t* cu_count, int64_t* hbm_bytes);
 

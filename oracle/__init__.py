@@ -36,6 +36,15 @@ class Props(C.Structure):
     ]
 
 
+class RefDist(C.Structure):
+    _fields_ = [
+        ("pos_min", C.c_double * 3), ("pos_max", C.c_double * 3),
+        ("vel_mean", C.c_double * 3), ("vel_stddev", C.c_double * 3),
+        ("num_particles", C.c_uint64), ("seed", C.c_uint32), ("pad_", C.c_int32),
+        ("q", C.c_double), ("qom", C.c_double),
+    ]
+
+
 def build() -> None:
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
@@ -91,6 +100,9 @@ def lib():
         L.or_mix64.argtypes = [U64]
         L.or_draw.argtypes = [U64, I32, I64, I64, P, P]
         L.or_init_species.argtypes = [P, I32, U64, P, C.c_double, C.c_double, P, P]
+        L.or_ref_init.argtypes = [P, C.POINTER(RefDist), P, P]
+        L.or_ref_normal_draws.argtypes = [U64, I64, P]
+        L.or_ref_uniform_draws.argtypes = [C.c_uint32, C.c_double, C.c_double, I64, P]
         _lib = L
     return _lib
 
@@ -105,6 +117,42 @@ def f64(a) -> np.ndarray:
 
 def i64(a) -> np.ndarray:
     return np.ascontiguousarray(np.asarray(a, dtype=np.int64))
+
+
+def ref_init(universe, pos_min, pos_max, vel_mean, vel_stddev, num_particles, q, qom, seed=0,
+             particles=True):
+    """initCells (cell.h:512-654) of distribution::uniform_pos_normal_speed with the reference's
+    random streams (oracle/ref_init.cpp): per-cell counts and, with particles=True, the
+    particles (PARTICLE_DTYPE, row-major cells) with q and qom set as initCells sets them."""
+    d = RefDist()
+    d.pos_min[:] = [float(v) for v in pos_min]
+    d.pos_max[:] = [float(v) for v in pos_max]
+    d.vel_mean[:] = [float(v) for v in vel_mean]
+    d.vel_stddev[:] = [float(v) for v in vel_stddev]
+    d.num_particles = int(num_particles)
+    d.seed = int(seed) & 0xFFFFFFFF
+    d.q, d.qom = float(q), float(qom)
+    counts = np.zeros(universe.ncell, dtype=np.int64)
+    rc = lib().or_ref_init(C.byref(universe.props), C.byref(d), _p(counts), None)
+    if rc != 0:
+        raise ValueError("initCells asserts for this particle count (no remainder to spread)")
+    if not particles:
+        return counts
+    out = np.zeros(int(counts.sum()), dtype=PARTICLE_DTYPE)
+    lib().or_ref_init(C.byref(universe.props), C.byref(d), _p(counts), _p(out))
+    return counts, out
+
+
+def ref_normal_draws(seed, n):
+    out = np.zeros(int(n), dtype=np.float32)
+    lib().or_ref_normal_draws(int(seed), int(n), _p(out))
+    return out
+
+
+def ref_uniform_draws(seed, a, b, n):
+    out = np.zeros(int(n), dtype=np.float64)
+    lib().or_ref_uniform_draws(int(seed) & 0xFFFFFFFF, float(a), float(b), int(n), _p(out))
+    return out
 
 
 class Universe:

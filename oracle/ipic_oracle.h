@@ -135,6 +135,22 @@ void or_draw(uint64_t seed, int32_t species, int64_t cell, int64_t index,
 int or_init_species(or_universe* u, int32_t species, uint64_t seed, const int64_t* counts,
                     double q, double qom, const double vth[3], const double drift[3]);
 
+/* ref_init.cpp: initCells (cell.h:512-654) over distribution::uniform_pos_normal_speed
+ * (cell.h:405-440) with the reference's random streams.  counts: ncell int64 (phase 1);
+ * out (nullable): sum(counts) particles, row-major cells, each cell's particles in draw order,
+ * with q and qom set as initCells sets them.  Returns -1 where initCells asserts. */
+typedef struct {
+    double pos_min[3], pos_max[3];
+    double vel_mean[3], vel_stddev[3];
+    uint64_t num_particles;
+    uint32_t seed;
+    int32_t pad_;
+    double q, qom;
+} or_refdist;
+int or_ref_init(const or_props* P, const or_refdist* D, int64_t* counts, or_particle* out);
+void or_ref_normal_draws(uint64_t seed, int64_t n, float* out);
+void or_ref_uniform_draws(uint32_t seed, double a, double b, int64_t n, double* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -10,6 +10,12 @@ Outputs (committed fixtures, data only):
                              (E x B drift, 5000 steps; simulation_test.cc:399-406)
   test_out_counts.txt     -- outputs/test.out: per-cell particle counts after the dipole-mode
                              run of inputs/test.inp (scripts/verify_output.sh.in:48 oracle)
+  tiny_out_counts.txt     -- outputs/tiny.out, the same for inputs/tiny.inp
+  test_conserved.txt      -- outputs/testConservedQuantities.out: per-cycle momentum, E and B
+  tiny_conserved.txt         field energies and kinetic energy of those runs (simulator.h:73-91);
+                             tiny_conserved.txt from outputs/tinyConservedQuantities.out
+  test_case.inp           -- the key = value lines of inputs/test.inp / tiny.inp that the
+  tiny_case.inp              driver reads (comments and unused keys dropped)
 Row format of the two trajectory files (as in the reference): step t x y z vx vy vz.
 """
 import os
@@ -29,10 +35,27 @@ def subset(src, dst, every):
 def main():
     subset("results/larmor_radius.txt", "larmor_radius_rows.txt", 50)
     subset("results/exb_drift.txt", "exb_drift_rows.txt", 100)
-    with open(os.path.join(REF, "outputs/test.out")) as f:
-        data = f.read()
-    with open(os.path.join(HERE, "test_out_counts.txt"), "w") as f:
-        f.write(data)
+    for src, dst in (("outputs/test.out", "test_out_counts.txt"), ("outputs/tiny.out", "tiny_out_counts.txt"),
+                     ("outputs/testConservedQuantities.out", "test_conserved.txt"),
+                     ("outputs/tinyConservedQuantities.out", "tiny_conserved.txt")):
+        with open(os.path.join(REF, src)) as f:
+            data = f.read()
+        with open(os.path.join(HERE, dst), "w") as f:
+            f.write(data)
+    keys = ("dt", "ncycles", "Lx", "Ly", "Lz", "nxc", "nyc", "nzc", "B0x", "B0y", "B0z", "B1x", "B1y", "B1z",
+            "npcelx", "npcely", "npcelz", "u0", "v0", "w0", "FieldOutputCycle", "ParticlesOutputCycle")
+    for case in ("test", "tiny"):
+        rows = []
+        with open(os.path.join(REF, "inputs", case + ".inp")) as f:
+            for ln in f:
+                body = ln.split("#")[0]
+                if "=" not in body:
+                    continue
+                k, v = (t.strip() for t in body.split("=", 1))
+                if k in keys:
+                    rows.append(f"{k} = {' '.join(v.split())}")
+        with open(os.path.join(HERE, case + "_case.inp"), "w") as f:
+            f.write("\n".join(rows) + "\n")
 
 
 if __name__ == "__main__":

@@ -1,0 +1,77 @@
+"""The reference's main driver case (src/ipic3d.cxx on inputs/test.inp) on the GPU: the
+library's reference-stream initialisation (ipic_gpu_init_reference) and 30 steps of the loop as
+shipped (analytic dipole mover, reflection, planet sink; FP32 on the device) against the FP64
+oracle run of the same case, step by step, and against the reference's outputs/test.out."""
+import os
+
+import numpy as np
+import pytest
+
+import allscale_ipic3d_amd as A
+from allscale_ipic3d_amd import driver as D, io
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _case():
+    return D.driver_case(D.Parameters(os.path.join(GOLD, "test_case.inp")), "test")
+
+
+def test_driver_case_matches_oracle_and_reference(tmp_path):
+    case = _case()
+    sim = D.make_simulation(case)
+    counts = sim.init_reference(0, D.case_distribution(case))
+    u = O.Universe(case.n, w=case.cell_width, origin=case.origin, dt=case.dt, c=case.c,
+                   sink_center=case.object_center, sink_radius=case.planet_radius, bc="reflect", mover="dipole",
+                   dipole_moment=case.dipole_moment, field_substeps=0)
+    ocounts, parts = O.ref_init(u, case.pos_min, case.pos_max, (0, 0, 0), (case.v_mod,) * 3, case.num_particles,
+                                *case.species)
+    u.add_particles(parts)
+    np.testing.assert_array_equal(counts, ocounts)
+    np.testing.assert_array_equal(sim.cell_counts(), u.cell_counts())
+    p = sim.download_particles(0)
+    # the device holds FP32 offsets in the cell and FP32 velocities of the same particles, in
+    # the same order (row-major cells, draw order within a cell)
+    np.testing.assert_allclose(p["pos"], parts["pos"], rtol=0, atol=2.0)  # metres, cells of 8e6 m
+    np.testing.assert_allclose(p["vel"], parts["vel"], rtol=1e-7, atol=0)
+    mass = case.species[0] / case.species[1]
+    worst = 0
+    for i in range(case.ncycles):
+        c = sim.conservation()
+        op, _ = u.particles()
+        v2 = (op["vel"] ** 2).sum(1)
+        assert abs(c["kinetic_energy"] / (0.5 * mass * v2.sum()) - 1) < 1e-5
+        assert abs(c["momentum"] / (mass * np.sqrt(v2)).sum() - 1) < 1e-5
+        sim.step(1)
+        u.move(8)
+        u.rebin(8)
+        gc, oc = sim.cell_counts(), u.cell_counts()
+        worst = max(worst, int(np.abs(gc - oc).sum()))
+        print(f"step {i + 1}: GPU {gc.sum()} oracle {oc.sum()} L1 {int(np.abs(gc - oc).sum())}", flush=True)
+    # FP32 against FP64 from the same state: a particle's cell may differ where it ends within
+    # the FP32 rounding of a face; at most 1e-4 of the particles
+    assert worst / 2 <= 1e-4 * case.num_particles
+    _, gold = io.read_cell_counts(os.path.join(GOLD, "test_out_counts.txt"))
+    final = sim.cell_counts()
+    assert abs(final.sum() / gold.sum() - 1) < 5e-4
+    assert np.abs(final - gold).sum() / 2 <= 0.01 * gold.sum()
+    sim.close()
+    u.close()
+
+
+def test_driver_run_writes_reference_formats(tmp_path):
+    res = D.run(os.path.join(GOLD, "test_case.inp"), out_dir=str(tmp_path))
+    n, counts = io.read_cell_counts(str(tmp_path / "test_case.out"))
+    assert n == (8, 8, 8)
+    np.testing.assert_array_equal(counts, res["counts"])
+    with open(tmp_path / "test_caseConservedQuantities.out") as f:
+        rows = f.read().splitlines()
+    with open(os.path.join(GOLD, "test_conserved.txt")) as f:
+        gold = f.read().splitlines()
+    assert rows[0] == gold[0] and len(rows) == len(gold)
+    for r, g in zip(rows[1:], gold[1:]):
+        r, g = r.split("\t"), g.split("\t")
+        assert r[0] == g[0] and r[2] == g[2] and r[3] == g[3]   # cycle, E and B energies
