@@ -7,6 +7,9 @@
 //   simulateSteps / simulateStep  simulator.h:42-57, 127-242, with GPU policy tags in place of
 //                               the per-cell projector / field solver / mover of :244-276
 //   Universe                    universe.h:12-51 (state resident on the GPU)
+//   distribution::uniform_pos_normal_speed, createUniverseFromDistribution, Universe::initCells
+//                               cell.h:405-440, universe.h:85-103, cell.h:512-654 (the
+//                               reference's own random streams)
 //
 // The reference aborts on a failed assert_*; here every failed C call throws
 // ipic3d_gpu::Error carrying the IPIC_E_* code and the library's message.  Header-only,
@@ -63,6 +66,22 @@ struct UniverseProperties {
     Coordinate3 tile{0, 0, 0};       // cells per GPU tile, 0: automatic
     std::uint32_t flags = 0;         // IPIC_FLAG_*
 };
+
+// the constants initCells gives every particle (cell.h:514-515, 644-645)
+constexpr double kElementaryCharge = 1.602176565e-19;
+constexpr double kProtonMass = 1.672621777e-27;
+
+namespace distribution {
+// uniform_pos_normal_speed (cell.h:405-440): positions uniform in [minPos, maxPos), velocities
+// center + stddev * the reference's ziggurat normal, per component
+struct uniform_pos_normal_speed {
+    Vector3 minPos{}, maxPos{}, center{}, stddev{};
+    std::uint32_t seed = 0;
+    uniform_pos_normal_speed(const Vector3& minPos_, const Vector3& maxPos_, const Vector3& center_,
+                             const Vector3& stddev_, std::uint32_t seed_ = 0)
+        : minPos(minPos_), maxPos(maxPos_), center(center_), stddev(stddev_), seed(seed_) {}
+};
+}  // namespace distribution
 
 // simulator.h:37-40
 struct DurationMeasurement {
@@ -176,6 +195,24 @@ class Universe {
         return out;
     }
 
+    // initCells (cell.h:512-654) over the distribution with the reference's random streams, into
+    // `species` (0: the first); returns the particles per cell (row-major)
+    std::vector<std::int64_t> initCells(std::uint64_t numParticles, const distribution::uniform_pos_normal_speed& d,
+                                        int species = 0) {
+        ipic_ref_distribution r{};
+        for (int k = 0; k < 3; ++k) {
+            r.pos_min[k] = d.minPos[k];
+            r.pos_max[k] = d.maxPos[k];
+            r.vel_mean[k] = d.center[k];
+            r.vel_stddev[k] = d.stddev[k];
+        }
+        r.num_particles = numParticles;
+        r.seed = d.seed;
+        std::vector<std::int64_t> counts((size_t)numCells());
+        check(ctx_, ipic_gpu_init_reference(ctx_, (int32_t)species, &r, counts.data()));
+        return counts;
+    }
+
     // countParticlesInDomain, cell.h:87-100
     std::int64_t countParticlesInDomain() const {
         int64_t n = 0;
@@ -232,6 +269,15 @@ class Universe {
     bool has_operator_ = false;
     ipic_step_stats last_{};
 };
+
+// createUniverseFromDistribution (universe.h:85-103): the particles initCells draws, all with
+// q = e and qom = e / m_p as there; the fields start at zero (Universe::setFields)
+inline Universe createUniverseFromDistribution(const UniverseProperties& props, std::uint64_t numParticles,
+                                               const distribution::uniform_pos_normal_speed& dist) {
+    Universe u(props, {{kElementaryCharge, kElementaryCharge / kProtonMass}});
+    u.initCells(numParticles, dist);
+    return u;
+}
 
 template <class ParticleToFieldProjector, class FieldSolver, class ParticleMover>
 DurationMeasurement simulateSteps(std::uint64_t numSteps, Universe& universe) {

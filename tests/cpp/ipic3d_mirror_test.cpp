@@ -5,6 +5,10 @@
 //                                                 writes <output.bin>.counts.out and
 //                                                 <output.bin>.log in the reference's formats
 //   ipic3d_mirror_test --expect-no-device          check the error path without a GPU
+//   ipic3d_mirror_test --driver-case <prefix>       the main driver's set-up of inputs/test.inp
+//                                                 (src/ipic3d.cxx:41-99): createUniverseFrom-
+//                                                 Distribution, 30 steps of the loop as shipped,
+//                                                 <prefix>.out in the reference's format
 //
 // input.bin (little-endian): int64 n, int64 steps, int64 nparticles, double dt, then per
 // particle 8 doubles (position, velocity, q, qom), then n^3 x 3 doubles E and n^3 x 3 Bc.
@@ -31,6 +35,43 @@ int main(int argc, char** argv) {
             std::printf("{\"error\": %d}\n", e.code());
             return e.code() == IPIC_E_HIP ? 0 : 1;
         }
+    }
+    if (argc == 3 && std::string(argv[1]) == "--driver-case") {
+        // ipic3d.cxx:47-85 with test.inp: 8^3 cells, L = 10, 8 x 8 x 8 particles per cell
+        const double R = 6378137.0, c = 299792458.0;
+        UniverseProperties p;
+        p.size = {8, 8, 8};
+        p.cellWidth = {10.0 / 8 * R, 10.0 / 8 * R, 10.0 / 8 * R};
+        p.dt = 0.01;
+        p.speedOfLight = c;
+        p.planetRadius = R;
+        p.objectCenter = {0.0, 0.0, 0.0};
+        Vector3 size{}, low{}, hig{};
+        for (int k = 0; k < 3; ++k) {
+            size[k] = p.cellWidth[k] * (double)p.size[k];
+            p.origin[k] = p.objectCenter[k] - size[k] / 2.0;
+            low[k] = p.origin[k] + 0.125 * size[k];
+            hig[k] = low[k] + 0.75 * size[k];
+        }
+        p.externalMagneticField = {0.0, 0.0, 3.07e-5};
+        p.shippedDipoleMover = true;
+        p.fieldSubsteps = 0;
+        p.flags = IPIC_FLAG_REFLECT;
+        const double K = 1e7 * kElementaryCharge, m = kProtonMass;
+        const double v_mod = c / std::sqrt(1.0 + (m * c * c) / K);
+        const std::uint64_t numParticles = 8 * 8 * 8 * 8 * 8 * 8;
+        try {
+            Universe u = createUniverseFromDistribution(
+                p, numParticles, distribution::uniform_pos_normal_speed(low, hig, {0, 0, 0}, {v_mod, v_mod, v_mod}));
+            const DurationMeasurement d = simulateSteps(30, u);
+            outputNumberOfParticlesPerCell(u, std::string(argv[2]) + ".out");
+            std::printf("{\"particles\": %lld, \"first\": %g, \"remaining\": %g}\n",
+                        (long long)u.countParticlesInDomain(), d.firstStep, d.remainingSteps);
+        } catch (const Error& e) {
+            std::fprintf(stderr, "error %d: %s\n", e.code(), e.what());
+            return 1;
+        }
+        return 0;
     }
     if (argc != 3) {
         std::fprintf(stderr, "usage: %s input.bin output.bin | --expect-no-device\n", argv[0]);

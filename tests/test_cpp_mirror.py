@@ -96,3 +96,19 @@ def test_cpp_mirror_matches_python_binding(tmp_path):
     assert rows[0].startswith("Cycle \t Total Moment") and len(rows) == 3
     assert rows[2].split("\t")[0] == str(steps - 1)
     sim.close()
+
+
+@pytest.mark.gpu
+@need_exe
+def test_cpp_mirror_driver_case(tmp_path):
+    """createUniverseFromDistribution + simulateSteps through the C++ mirror on the main driver's
+    test.inp set-up gives the per-cell counts of the Python driver on the same case."""
+    from allscale_ipic3d_amd import driver as D, io as IO
+    prefix = str(tmp_path / "cpp_test")
+    r = subprocess.run([EXE, "--driver-case", prefix], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    n, cells = IO.read_cell_counts(prefix + ".out")
+    assert n == (8, 8, 8)
+    res = D.run(os.path.join(ROOT, "tests", "golden", "test_case.inp"), write=False)
+    np.testing.assert_array_equal(cells, res["counts"])
