@@ -737,10 +737,11 @@ __global__ __launch_bounds__(NTP) IPIC_PUSH_ATTR void k_particles(PushArgs A) {
                 bool leaver = false;
                 float px = rx[jj], py = ry[jj], pz = rz[jj];
                 float vx = rvx[jj], vy = rvy[jj], vz = rvz[jj];
-                if (alive(px, py, pz)) {
-                    if (!in_cell(px, py, pz)) {
-                        qe = (pst + jj - plo) | (k << 9);  // sits in another cell
-                    } else {
+                const bool live = alive(px, py, pz);
+                const bool home = live && in_cell(px, py, pz);  // in the run's cell: fast path
+                if (live && !home) qe = (pst + jj - plo) | (k << 9);  // sits in another cell
+                {
+                    if (home) {
                         bool sunk = false;
                         if (MODE == MODE_PUSH) {
                             float Ex, Ey, Ez, Bx, By, Bz;

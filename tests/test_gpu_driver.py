@@ -75,3 +75,33 @@ def test_driver_run_writes_reference_formats(tmp_path):
     for r, g in zip(rows[1:], gold[1:]):
         r, g = r.split("\t"), g.split("\t")
         assert r[0] == g[0] and r[2] == g[2] and r[3] == g[3]   # cycle, E and B energies
+
+
+def test_tiny_case_matches_oracle_and_reference():
+    """inputs/tiny.inp: 16^3 cells x 512 particles (2,097,152), 100 steps, against the oracle
+    every 25 steps and against outputs/tiny.out at the end."""
+    case = D.driver_case(D.Parameters(os.path.join(GOLD, "tiny_case.inp")), "tiny")
+    sim = D.make_simulation(case)
+    counts = sim.init_reference(0, D.case_distribution(case))
+    u = O.Universe(case.n, w=case.cell_width, origin=case.origin, dt=case.dt, c=case.c,
+                   sink_center=case.object_center, sink_radius=case.planet_radius, bc="reflect", mover="dipole",
+                   dipole_moment=case.dipole_moment, field_substeps=0)
+    ocounts, parts = O.ref_init(u, case.pos_min, case.pos_max, (0, 0, 0), (case.v_mod,) * 3, case.num_particles,
+                                *case.species)
+    u.add_particles(parts)
+    np.testing.assert_array_equal(counts, ocounts)
+    for block in range(4):
+        sim.step(25)
+        for _ in range(25):
+            u.move(0)
+            u.rebin(0)
+        gc, oc = sim.cell_counts(), u.cell_counts()
+        moved = int(np.abs(gc - oc).sum()) // 2
+        print(f"step {25 * (block + 1)}: GPU {gc.sum()} oracle {oc.sum()} in another cell {moved}", flush=True)
+        assert moved <= 1e-4 * case.num_particles
+    _, gold = io.read_cell_counts(os.path.join(GOLD, "tiny_out_counts.txt"))
+    final = sim.cell_counts()
+    assert abs(final.sum() / gold.sum() - 1) < 5e-4
+    assert np.abs(final - gold).sum() / 2 <= 0.01 * gold.sum()
+    sim.close()
+    u.close()
