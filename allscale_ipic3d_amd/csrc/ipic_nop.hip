@@ -5,13 +5,14 @@
 //
 // Activations are channel-last on the n^3 index lattice: X[v * C + c], v = (x * ny + y) * nz + z.
 //
-// Forward convolution (k_conv): a block computes a 4x4x16 brick of output voxels for every
-// output channel.  The brick's input halo (6x6x18 voxels x CIN channels) is staged in LDS with
-// an 8-byte pad per voxel (conflict-free A-fragment reads); each wave owns one x-plane of the
-// brick, i.e. four 16-voxel z-rows, and for every tap and 16-channel slice issues one MFMA per
-// (row, 16-output-channel tile): the A operand is 16 voxels x 16 input channels from LDS, the
-// B operand 16 input x 16 output channels of the tap's weights from global memory (L1/L2
-// resident).  The epilogue adds the bias, optionally multiplies by silu'(z_saved) and adds a
+// Forward convolution (k_conv): one persistent block per CU stages the layer's weights in LDS
+// once and walks 4x4x16 bricks of output voxels, computing every output channel of a brick.  The
+// brick's input halo (6x6x18 voxels x CIN channels) is staged in LDS with an 8-byte pad per
+// voxel (conflict-free A-fragment reads; the weight rows carry the same pad), and the next
+// brick's halo is loaded into registers while the current brick computes.  Each wave owns one
+// x-plane of the brick, i.e. four 16-voxel z-rows, and for every tap and 16-channel slice issues
+// one MFMA per (row, 16-output-channel tile): the A operand is 16 voxels x 16 input channels,
+// the B operand 16 input x 16 output channels of the tap's weights, both from LDS.  The epilogue adds the bias, optionally multiplies by silu'(z_saved) and adds a
 // residual, and writes FP32 and/or BF16 (optionally silu-activated) outputs.
 //
 // Data gradient: the same kernel with the weights mirrored in the taps and transposed in the
@@ -59,10 +60,12 @@ __device__ __forceinline__ uint16_t f2bf(float f) {
     return (uint16_t)(u >> 16);
 }
 
-__device__ __forceinline__ float silu(float x) { return x / (1.0f + __expf(-x)); }
+// silu and its derivative with the hardware reciprocal (1 ulp): the results feed BF16 values
+// and an FP32 residual stream whose error budget is set by the BF16 rounding of the weights
+__device__ __forceinline__ float silu(float x) { return x * __builtin_amdgcn_rcpf(1.0f + __expf(-x)); }
 
 __device__ __forceinline__ float dsilu(float x) {
-    const float s = 1.0f / (1.0f + __expf(-x));
+    const float s = __builtin_amdgcn_rcpf(1.0f + __expf(-x));
     return s * (1.0f + x * (1.0f - s));
 }
 
@@ -81,74 +84,132 @@ struct ConvArgs {
 };
 
 template <int CIN, int COUT>
-__global__ __launch_bounds__(256) void k_conv(ConvArgs a) {
-    constexpr int SV = CIN * 2 + 8;  // LDS bytes per halo voxel
+__global__ __launch_bounds__(256) void k_conv(ConvArgs a, int nbricks) {
+    constexpr int SV = CIN * 2 + 8;  // LDS bytes per halo voxel and per weight row
     constexpr int PV = CIN / 4;      // 8-byte pieces per voxel
     constexpr int NT = COUT / 16;
-    __shared__ __align__(16) unsigned char sx[HV * SV];
+    constexpr int NP = HV * PV;      // halo pieces of a brick
+    constexpr int NH = (NP + 255) / 256;
+    __shared__ __align__(16) unsigned char sw[27 * COUT * SV];  // the layer's weights, all bricks
+    __shared__ __align__(16) unsigned char sx[HV * SV];         // the current brick's input halo
     const int n0 = a.n[0], n1 = a.n[1], n2 = a.n[2];
     const int nbz = n2 / BZ, nby = n1 / BY;
-    const int b = blockIdx.x;
-    const int bz = b % nbz, by = (b / nbz) % nby, bx = b / (nbz * nby);
-    const int x0 = bx * BX, y0 = by * BY, z0 = bz * BZ;
 
-    for (int i = threadIdx.x; i < HV * PV; i += 256) {
+    for (int i = threadIdx.x; i < 27 * COUT * PV; i += 256) {
+        const int row = i / PV, p = i - row * PV;
+        *(uint2*)(sw + row * SV + p * 8) = *(const uint2*)(a.w + (long long)row * CIN + p * 4);
+    }
+
+    // the next brick's halo travels in registers while the current brick computes
+    // this thread's halo pieces: voxel offsets (hx, hy, hz) in the halo and 8-byte piece p,
+    // packed as hx | hy << 4 | hz << 8 | p << 16
+    int piece[NH];
+#pragma unroll
+    for (int j = 0; j < NH; ++j) {
+        const int i = threadIdx.x + j * 256;
         const int v = i / PV, p = i - v * PV;
         const int hx = v / (HY * HZ), r = v - hx * (HY * HZ), hy = r / HZ, hz = r - hy * HZ;
-        const int gx = wrap1(x0 + hx - 1, n0), gy = wrap1(y0 + hy - 1, n1), gz = wrap1(z0 + hz - 1, n2);
-        const long long gv = ((long long)gx * n1 + gy) * n2 + gz;
-        *(uint2*)(sx + v * SV + p * 8) = *(const uint2*)(a.x + gv * CIN + p * 4);
+        piece[j] = i < NP ? (hx | hy << 4 | hz << 8 | p << 16) : -1;
     }
-    __syncthreads();
+    uint2 pre[NH];
+    auto fetch = [&](int bb) {
+        const int bz = bb % nbz, by = (bb / nbz) % nby, bx = bb / (nbz * nby);
+#pragma unroll
+        for (int j = 0; j < NH; ++j) {
+            if (piece[j] >= 0) {
+                const int hx = piece[j] & 15, hy = (piece[j] >> 4) & 15, hz = (piece[j] >> 8) & 255;
+                const int p = piece[j] >> 16;
+                const int gx = wrap1(bx * BX + hx - 1, n0), gy = wrap1(by * BY + hy - 1, n1);
+                const int gz = wrap1(bz * BZ + hz - 1, n2);
+                const long long gv = ((long long)gx * n1 + gy) * n2 + gz;
+                pre[j] = *(const uint2*)(a.x + gv * CIN + p * 4);
+            }
+        }
+    };
 
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int m = lane & 15, kq = lane >> 4;
-    v4f acc[BY][NT];
+    if ((int)blockIdx.x < nbricks) fetch(blockIdx.x);
+    for (int bb = blockIdx.x; bb < nbricks; bb += gridDim.x) {
+        const int bz = bb % nbz, by = (bb / nbz) % nby, bx = bb / (nbz * nby);
+        const int x0 = bx * BX, y0 = by * BY, z0 = bz * BZ;
+        __syncthreads();  // the previous brick's halo reads are done (and the weights are in)
 #pragma unroll
-    for (int y = 0; y < BY; ++y)
-#pragma unroll
-        for (int t = 0; t < NT; ++t) acc[y][t] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
-
-    for (int dx = 0; dx < 3; ++dx)
-        for (int dy = 0; dy < 3; ++dy) {
-#pragma unroll
-            for (int dz = 0; dz < 3; ++dz) {
-                const int tap = (dx * 3 + dy) * 3 + dz;
-#pragma unroll
-                for (int s = 0; s < CIN / 16; ++s) {
-                    v4s bf[NT];
-#pragma unroll
-                    for (int t = 0; t < NT; ++t)
-                        bf[t] = *(const v4s*)(a.w + ((tap * COUT + t * 16 + m) * CIN + s * 16 + kq * 4));
-#pragma unroll
-                    for (int y = 0; y < BY; ++y) {
-                        const int hv = ((w + dx) * HY + (y + dy)) * HZ + (m + dz);
-                        const v4s af = *(const v4s*)(sx + hv * SV + (s * 16 + kq * 4) * 2);
-#pragma unroll
-                        for (int t = 0; t < NT; ++t)
-                            acc[y][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af, bf[t], acc[y][t], 0, 0, 0);
-                    }
-                }
+        for (int j = 0; j < NH; ++j) {
+            const int i = threadIdx.x + j * 256;
+            if (i < NP) {
+                const int v = i / PV, p = i - v * PV;
+                *(uint2*)(sx + v * SV + p * 8) = pre[j];
             }
         }
-
-    // D[row = z (4 kq + j)][col = output channel (t 16 + m)]
+        __syncthreads();
+        if (bb + (int)gridDim.x < nbricks) fetch(bb + gridDim.x);
+        // the epilogue's inputs, loaded now so that their latency hides behind the MFMAs
+        float ez[NT][BY][4], er[NT][BY][4];
+        const long long ob = (((long long)(x0 + w) * n1 + y0) * n2 + z0 + kq * 4) * COUT + m;  // (t, y, j) = 0
+        if (a.zs) {
 #pragma unroll
-    for (int t = 0; t < NT; ++t) {
-        const int co = t * 16 + m;
-        const float bb = a.bias ? a.bias[co] : 0.0f;
+            for (int t = 0; t < NT; ++t)
+#pragma unroll
+                for (int y = 0; y < BY; ++y)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) ez[t][y][j] = a.zs[ob + ((long long)y * n2 + j) * COUT + t * 16];
+        }
+        if (a.res) {
+#pragma unroll
+            for (int t = 0; t < NT; ++t)
+#pragma unroll
+                for (int y = 0; y < BY; ++y)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) er[t][y][j] = a.res[ob + ((long long)y * n2 + j) * COUT + t * 16];
+        }
+
+        v4f acc[BY][NT];
 #pragma unroll
         for (int y = 0; y < BY; ++y)
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const long long gv = ((long long)(x0 + w) * n1 + (y0 + y)) * n2 + (z0 + kq * 4 + j);
-                const long long o = gv * COUT + co;
-                float v = acc[y][t][j] + bb;
-                if (a.zs) v *= dsilu(a.zs[o]);
-                if (a.res) v += a.res[o];
-                if (a.outf) a.outf[o] = v;
-                if (a.outb) a.outb[o] = f2bf(a.act ? silu(v) : v);
+            for (int t = 0; t < NT; ++t) acc[y][t] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+
+        for (int dx = 0; dx < 3; ++dx)
+            for (int dy = 0; dy < 3; ++dy) {
+#pragma unroll
+                for (int dz = 0; dz < 3; ++dz) {
+                    const int tap = (dx * 3 + dy) * 3 + dz;
+#pragma unroll
+                    for (int s2 = 0; s2 < CIN / 16; ++s2) {
+                        v4s bf[NT];
+#pragma unroll
+                        for (int t = 0; t < NT; ++t)
+                            bf[t] = *(const v4s*)(sw + (tap * COUT + t * 16 + m) * SV + (s2 * 16 + kq * 4) * 2);
+#pragma unroll
+                        for (int y = 0; y < BY; ++y) {
+                            const int hv = ((w + dx) * HY + (y + dy)) * HZ + (m + dz);
+                            const v4s af = *(const v4s*)(sx + hv * SV + (s2 * 16 + kq * 4) * 2);
+#pragma unroll
+                            for (int t = 0; t < NT; ++t)
+                                acc[y][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af, bf[t], acc[y][t], 0, 0, 0);
+                        }
+                    }
+                }
             }
+
+        // D[row = z (4 kq + j)][col = output channel (t 16 + m)]
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const int co = t * 16 + m;
+            const float bias = a.bias ? a.bias[co] : 0.0f;
+#pragma unroll
+            for (int y = 0; y < BY; ++y)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const long long o = ob + ((long long)y * n2 + j) * COUT + t * 16;
+                    float v = acc[y][t][j] + bias;
+                    if (a.zs) v *= dsilu(ez[t][y][j]);
+                    if (a.res) v += er[t][y][j];
+                    if (a.outf) a.outf[o] = v;
+                    if (a.outb) a.outb[o] = f2bf(a.act ? silu(v) : v);
+                }
+        }
     }
 }
 
@@ -480,12 +541,27 @@ int check_grid(const int32_t n[3]) {
         if (e_ != hipSuccess) return fail(IPIC_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
+// compute units of the current device (cached per device)
+int cu_count() {
+    static int cached[64] = {0};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+    if (!cached[dev]) {
+        int c = 0;
+        if (hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || c <= 0) c = 256;
+        cached[dev] = c;
+    }
+    return cached[dev];
+}
+
 template <int CIN, int COUT>
 int conv(const int32_t n[3], const uint16_t* x, const uint16_t* w, const float* bias, const float* zs,
          const float* res, float* outf, uint16_t* outb, int act, hipStream_t st) {
     ConvArgs a{{n[0], n[1], n[2]}, x, w, bias, zs, res, outf, outb, act};
-    const int blocks = (n[0] / BX) * (n[1] / BY) * (n[2] / BZ);
-    hipLaunchKernelGGL((k_conv<CIN, COUT>), dim3(blocks), dim3(256), 0, st, a);
+    const int bricks = (n[0] / BX) * (n[1] / BY) * (n[2] / BZ);
+    // one persistent block per CU: the weights are staged in LDS once per block
+    const int blocks = bricks < cu_count() ? bricks : cu_count();
+    hipLaunchKernelGGL((k_conv<CIN, COUT>), dim3(blocks), dim3(256), 0, st, a, bricks);
     NOPCK(hipGetLastError());
     return 0;
 }
