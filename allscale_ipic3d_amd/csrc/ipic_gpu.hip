@@ -120,6 +120,7 @@ struct PushArgs {
     int* jflag;
     unsigned long long* removed;
     int* err;             // set to nonzero when a consistency guard fires
+    const int* tile_order;  // block -> tile (a permutation), or null for the XCD-aware default
 };
 
 // ---------------------------------------------------------------------------------------------
@@ -590,7 +591,7 @@ __global__ __launch_bounds__(NTP) IPIC_PUSH_ATTR void k_particles(PushArgs A) {
     unsigned short* Q = (unsigned short*)(wbase + (WS + 1) * 4);     // arrival queue
     float* K0 = (float*)(wbase + (((WS + 1) * 4 + WCH * 2 + 15) & ~15));  // leaver buffer
 
-    const int t = xcd_tile(blockIdx.x, g.ntiles);
+    const int t = A.tile_order ? A.tile_order[blockIdx.x] : xcd_tile(blockIdx.x, g.ntiles);
     const int tx = t / (g.NTl[1] * g.NTl[2]), ty = (t / g.NTl[2]) % g.NTl[1], tz = t % g.NTl[2];
     const int x0 = tx * G.T0, y0 = ty * G.T1, z0 = tz * G.T2;
     const int cbase = t * G.TC;
@@ -1219,6 +1220,65 @@ __global__ __launch_bounds__(NT) void k_scan_cells(ScanArgs A) {
     }
 }
 
+// Launch order of the particle kernel's tiles for the layout the scans just produced.  A tile
+// costs time in proportion to its particles, and one block works one tile, so with an uneven
+// load the heaviest tiles go first (longest-processing-time order, by 256 load classes); with
+// a load within 25 % of the mean the XCD-aware order stays.  Any permutation gives the same
+// results: the order only schedules.
+__global__ __launch_bounds__(1024) void k_tile_order(ScanArgs A, int nsp, int* order) {
+    const Geo& g = A.g;
+    __shared__ unsigned long long s_sum[16];
+    __shared__ int s_max[16];
+    __shared__ int s_hist[256];
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    auto load = [&](int t) {
+        int v = 0;
+        for (int sp = 0; sp < nsp; ++sp) v += A.sp[sp].tile_total[t];
+        return v;
+    };
+    unsigned long long sum = 0;
+    int mx = 0;
+    for (int t = tid; t < g.ntiles; t += 1024) {
+        const int v = load(t);
+        sum += (unsigned long long)v;
+        mx = max(mx, v);
+    }
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) {
+        sum += __shfl_xor(sum, d, 64);
+        mx = max(mx, __shfl_xor(mx, d, 64));
+    }
+    if (lane == 0) {
+        s_sum[wid] = sum;
+        s_max[wid] = mx;
+    }
+    if (tid < 256) s_hist[tid] = 0;
+    __syncthreads();
+    sum = 0;
+    mx = 0;
+    for (int w = 0; w < 16; ++w) {
+        sum += s_sum[w];
+        mx = max(mx, s_max[w]);
+    }
+    if ((double)mx * g.ntiles <= 1.25 * (double)sum || mx == 0) {
+        for (int b = tid; b < g.ntiles; b += 1024) order[b] = xcd_tile(b, g.ntiles);
+        return;
+    }
+    auto cls = [&](int v) { return 255 - (int)(((long long)v * 255) / mx); };  // heaviest: class 0
+    for (int t = tid; t < g.ntiles; t += 1024) atomicAdd(&s_hist[cls(load(t))], 1);
+    __syncthreads();
+    if (tid == 0) {
+        int run = 0;
+        for (int k = 0; k < 256; ++k) {
+            const int c = s_hist[k];
+            s_hist[k] = run;
+            run += c;
+        }
+    }
+    __syncthreads();
+    for (int t = tid; t < g.ntiles; t += 1024) order[atomicAdd(&s_hist[cls(load(t))], 1)] = t;
+}
+
 // After a REBIN pass the array is sorted by current cell: segments = starts, all stay.
 __global__ __launch_bounds__(256) void k_canon_fix(ScanArgs A) {
     const Geo& g = A.g;
@@ -1494,6 +1554,8 @@ struct ipic_ctx {
     int* jpairs = nullptr;
     unsigned long long* removed = nullptr;
     int* derr = nullptr;
+    int* tile_order = nullptr;     // launch order of the particle kernel's tiles
+    bool order_valid = false;      // tile_order describes the current layout's loads
     double* red = nullptr;
     int red_blocks = 1024;
     Species sp[IPIC_MAX_SPECIES];
@@ -1610,6 +1672,7 @@ PushArgs push_args(ipic_ctx* ctx) {
     A.jflag = ctx->jflag;
     A.removed = ctx->removed;
     A.err = ctx->derr;
+    A.tile_order = ctx->order_valid ? ctx->tile_order : nullptr;
     return A;
 }
 
@@ -1713,7 +1776,9 @@ int launch_scan(ipic_ctx* ctx) {
     hipLaunchKernelGGL(k_scan_tiles, dim3(ctx->g.ntiles, ns), dim3(NT), 0, ctx->stream, A);
     hipLaunchKernelGGL(k_scan_top, dim3(ns), dim3(1024), 0, ctx->stream, A);
     hipLaunchKernelGGL(k_scan_cells, dim3(ctx->g.ntiles, ns), dim3(NT), 0, ctx->stream, A);
+    hipLaunchKernelGGL(k_tile_order, dim3(1), dim3(1024), 0, ctx->stream, A, ns, ctx->tile_order);
     HIPCK(ctx, hipGetLastError());
+    ctx->order_valid = true;
     return 0;
 }
 
@@ -1973,6 +2038,7 @@ int alloc_species_buffers(ipic_ctx* ctx, int s, long long cap) {
 // set the per-cell bookkeeping of a freshly sorted (canonical) array from host starts
 int set_layout(ipic_ctx* ctx, int s, const std::vector<int>& start) {
     Species& S = ctx->sp[s];
+    ctx->order_valid = false;  // the tile loads of the new layout are not known on the device
     const int nc = ctx->g.ncell;
     std::vector<int> stay(nc);
     for (int c = 0; c < nc; ++c) stay[c] = start[c + 1] - start[c];
@@ -2148,6 +2214,7 @@ int ipic_gpu_create(const ipic_desc* desc, ipic_ctx** out) {
     rc = rc ? rc : dalloc(ctx, &ctx->jpairs, jp.size());
     rc = rc ? rc : dalloc(ctx, &ctx->removed, 1);
     rc = rc ? rc : dalloc(ctx, &ctx->derr, 1);
+    rc = rc ? rc : dalloc(ctx, &ctx->tile_order, (size_t)g.ntiles);
     rc = rc ? rc : dalloc(ctx, &ctx->red, (size_t)ctx->red_blocks * 5);
     for (int s = 0; s < d.nspecies && !rc; ++s) {
         Species& S = ctx->sp[s];

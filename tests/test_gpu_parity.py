@@ -342,6 +342,44 @@ def test_config3_dipole_drift_uneven_load_small():
     u.close()
 
 
+def test_full_size_invariants_config3():
+    """The uneven-load configuration at its full size: 128^3 cells, a centred z dipole (moment 1)
+    as the magnetic background, drift 0.1 along x, 30 (1 + 9 exp(-r^2 / 20^2)) particles per
+    cell (30 to 299, 74.9M particles), a sink of radius 4 cells at the centre.  Size-independent
+    checks: the deposit's partition of unity sum(J) 8 vol = sum(q v), particles = initial -
+    removed, every particle inside its cell, div+B unchanged."""
+    import allscale_ipic3d_amd as G
+    n = (128, 128, 128)
+    B, c = _dipole_centres(n, radius=4.0)
+    x, y, z = np.meshgrid(*[np.arange(k) + 0.5 for k in n], indexing="ij")
+    r2 = ((x - c[0]) ** 2 + (y - c[1]) ** 2 + (z - c[2]) ** 2).reshape(-1)
+    ppc = np.rint(30 * (1 + 9 * np.exp(-r2 / 400.0))).astype(np.int32)
+    q = -1.0 / (30 * 4 * np.pi)
+    sim = G.Simulation(n, dt=0.1, species=((q, -1.0),), sink_center=tuple(c), sink_radius=4.0)
+    sim.init_particles(0, ppc, (0.05,) * 3, drift=(0.1, 0.0, 0.0), seed=3)
+    sim.upload_fields(np.zeros((n[0] * n[1] * n[2], 3)), B)
+    total = int(ppc.sum())
+    print("particles", total, "ppc", int(ppc.min()), int(ppc.max()), flush=True)
+    assert ppc.min() == 30 and ppc.max() >= 299 and total > 70_000_000
+    qv = q * sim.download_particles(0)["vel"].sum(0)
+    c0 = sim.conservation()
+    st = sim.step(1)
+    J = sim.download_fields()["J"]
+    np.testing.assert_allclose(J.sum(0) * 8.0, qv, rtol=1e-4, atol=1e-9 * np.abs(qv).max())
+    st2 = sim.step(4)
+    c1 = sim.conservation()
+    removed = st["removed"] + st2["removed"]
+    print("removed", removed, "ms per step", 1e3 * st2["remaining_steps_s"] / 3, flush=True)
+    assert removed > 0
+    assert c1["particles"] == total - removed
+    assert abs(c1["divB_l2"] - c0["divB_l2"]) <= 1e-6 * max(1.0, c0["divB_l2"])
+    p = sim.download_particles(0)
+    off = p["pos"] - np.stack(np.unravel_index(p["cell"], n), -1)
+    assert off.min() >= 0.0 and off.max() <= 1.0
+    assert np.isfinite(p["vel"]).all()
+    sim.close()
+
+
 def test_reflecting_faces_match_oracle():
     """Reflection at the global faces (cell.h:1103-1121, IPIC_FLAG_REFLECT) against the
     oracle's reflecting rebin: fast particles near every face of a small box, 8 steps."""
