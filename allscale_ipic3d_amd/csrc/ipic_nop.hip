@@ -84,41 +84,34 @@ struct ConvArgs {
 };
 
 template <int CIN, int COUT>
-__global__ __launch_bounds__(256) void k_conv(ConvArgs a, int nbricks) {
+__global__ __launch_bounds__(512) void k_conv(ConvArgs a, int nbricks) {
     constexpr int SV = CIN * 2 + 8;  // LDS bytes per halo voxel and per weight row
     constexpr int PV = CIN / 4;      // 8-byte pieces per voxel
     constexpr int NT = COUT / 16;
     constexpr int NP = HV * PV;      // halo pieces of a brick
     constexpr int NH = (NP + 255) / 256;
     __shared__ __align__(16) unsigned char sw[27 * COUT * SV];  // the layer's weights, all bricks
-    __shared__ __align__(16) unsigned char sx[HV * SV];         // the current brick's input halo
+    __shared__ __align__(16) unsigned char sx[2][HV * SV];      // the two current bricks' halos
     const int n0 = a.n[0], n1 = a.n[1], n2 = a.n[2];
     const int nbz = n2 / BZ, nby = n1 / BY;
+    // the block's two halves (4 waves each) work on bricks 2 p and 2 p + 1 of pair p
+    const int half = threadIdx.x >> 8, ht = threadIdx.x & 255;
 
-    for (int i = threadIdx.x; i < 27 * COUT * PV; i += 256) {
+    for (int i = threadIdx.x; i < 27 * COUT * PV; i += 512) {
         const int row = i / PV, p = i - row * PV;
         *(uint2*)(sw + row * SV + p * 8) = *(const uint2*)(a.w + (long long)row * CIN + p * 4);
     }
 
     // the next brick's halo travels in registers while the current brick computes
-    // this thread's halo pieces: voxel offsets (hx, hy, hz) in the halo and 8-byte piece p,
-    // packed as hx | hy << 4 | hz << 8 | p << 16
-    int piece[NH];
-#pragma unroll
-    for (int j = 0; j < NH; ++j) {
-        const int i = threadIdx.x + j * 256;
-        const int v = i / PV, p = i - v * PV;
-        const int hx = v / (HY * HZ), r = v - hx * (HY * HZ), hy = r / HZ, hz = r - hy * HZ;
-        piece[j] = i < NP ? (hx | hy << 4 | hz << 8 | p << 16) : -1;
-    }
     uint2 pre[NH];
     auto fetch = [&](int bb) {
         const int bz = bb % nbz, by = (bb / nbz) % nby, bx = bb / (nbz * nby);
 #pragma unroll
         for (int j = 0; j < NH; ++j) {
-            if (piece[j] >= 0) {
-                const int hx = piece[j] & 15, hy = (piece[j] >> 4) & 15, hz = (piece[j] >> 8) & 255;
-                const int p = piece[j] >> 16;
+            const int i = ht + j * 256;
+            if (i < NP) {
+                const int v = i / PV, p = i - v * PV;
+                const int hx = v / (HY * HZ), r = v - hx * (HY * HZ), hy = r / HZ, hz = r - hy * HZ;
                 const int gx = wrap1(bx * BX + hx - 1, n0), gy = wrap1(by * BY + hy - 1, n1);
                 const int gz = wrap1(bz * BZ + hz - 1, n2);
                 const long long gv = ((long long)gx * n1 + gy) * n2 + gz;
@@ -127,42 +120,30 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs a, int nbricks) {
         }
     };
 
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, w = (threadIdx.x >> 6) & 3;
     const int m = lane & 15, kq = lane >> 4;
-    if ((int)blockIdx.x < nbricks) fetch(blockIdx.x);
-    for (int bb = blockIdx.x; bb < nbricks; bb += gridDim.x) {
-        const int bz = bb % nbz, by = (bb / nbz) % nby, bx = bb / (nbz * nby);
-        const int x0 = bx * BX, y0 = by * BY, z0 = bz * BZ;
-        __syncthreads();  // the previous brick's halo reads are done (and the weights are in)
+    unsigned char* hx_ = sx[half];
+    if (2 * (int)blockIdx.x + half < nbricks) fetch(2 * blockIdx.x + half);
+    for (int pr = blockIdx.x; 2 * pr < nbricks; pr += gridDim.x) {
+        const int bb = 2 * pr + half;
+        const bool mine = bb < nbricks;
+        __syncthreads();  // the previous bricks' halo reads are done (and the weights are in)
+        if (mine) {
 #pragma unroll
-        for (int j = 0; j < NH; ++j) {
-            const int i = threadIdx.x + j * 256;
-            if (i < NP) {
-                const int v = i / PV, p = i - v * PV;
-                *(uint2*)(sx + v * SV + p * 8) = pre[j];
+            for (int j = 0; j < NH; ++j) {
+                const int i = ht + j * 256;
+                if (i < NP) {
+                    const int v = i / PV, p = i - v * PV;
+                    *(uint2*)(hx_ + v * SV + p * 8) = pre[j];
+                }
             }
         }
         __syncthreads();
-        if (bb + (int)gridDim.x < nbricks) fetch(bb + gridDim.x);
-        // the epilogue's inputs, loaded now so that their latency hides behind the MFMAs
-        float ez[NT][BY][4], er[NT][BY][4];
-        const long long ob = (((long long)(x0 + w) * n1 + y0) * n2 + z0 + kq * 4) * COUT + m;  // (t, y, j) = 0
-        if (a.zs) {
-#pragma unroll
-            for (int t = 0; t < NT; ++t)
-#pragma unroll
-                for (int y = 0; y < BY; ++y)
-#pragma unroll
-                    for (int j = 0; j < 4; ++j) ez[t][y][j] = a.zs[ob + ((long long)y * n2 + j) * COUT + t * 16];
-        }
-        if (a.res) {
-#pragma unroll
-            for (int t = 0; t < NT; ++t)
-#pragma unroll
-                for (int y = 0; y < BY; ++y)
-#pragma unroll
-                    for (int j = 0; j < 4; ++j) er[t][y][j] = a.res[ob + ((long long)y * n2 + j) * COUT + t * 16];
-        }
+        const int nb = 2 * (pr + (int)gridDim.x) + half;
+        if (nb < nbricks) fetch(nb);
+        if (!mine) continue;
+        const int bz = bb % nbz, by = (bb / nbz) % nby, bx = bb / (nbz * nby);
+        const int x0 = bx * BX, y0 = by * BY, z0 = bz * BZ;
 
         v4f acc[BY][NT];
 #pragma unroll
@@ -184,7 +165,7 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs a, int nbricks) {
 #pragma unroll
                         for (int y = 0; y < BY; ++y) {
                             const int hv = ((w + dx) * HY + (y + dy)) * HZ + (m + dz);
-                            const v4s af = *(const v4s*)(sx + hv * SV + (s2 * 16 + kq * 4) * 2);
+                            const v4s af = *(const v4s*)(hx_ + hv * SV + (s2 * 16 + kq * 4) * 2);
 #pragma unroll
                             for (int t = 0; t < NT; ++t)
                                 acc[y][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af, bf[t], acc[y][t], 0, 0, 0);
@@ -194,18 +175,18 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs a, int nbricks) {
             }
 
         // D[row = z (4 kq + j)][col = output channel (t 16 + m)]
+        const long long ob = (((long long)(x0 + w) * n1 + y0) * n2 + z0 + kq * 4) * COUT + m;  // (t, y, j) = 0
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
-            const int co = t * 16 + m;
-            const float bias = a.bias ? a.bias[co] : 0.0f;
+            const float bias = a.bias ? a.bias[t * 16 + m] : 0.0f;
 #pragma unroll
             for (int y = 0; y < BY; ++y)
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
                     const long long o = ob + ((long long)y * n2 + j) * COUT + t * 16;
                     float v = acc[y][t][j] + bias;
-                    if (a.zs) v *= dsilu(ez[t][y][j]);
-                    if (a.res) v += er[t][y][j];
+                    if (a.zs) v *= dsilu(a.zs[o]);
+                    if (a.res) v += a.res[o];
                     if (a.outf) a.outf[o] = v;
                     if (a.outb) a.outb[o] = f2bf(a.act ? silu(v) : v);
                 }
@@ -559,9 +540,11 @@ int conv(const int32_t n[3], const uint16_t* x, const uint16_t* w, const float* 
          const float* res, float* outf, uint16_t* outb, int act, hipStream_t st) {
     ConvArgs a{{n[0], n[1], n[2]}, x, w, bias, zs, res, outf, outb, act};
     const int bricks = (n[0] / BX) * (n[1] / BY) * (n[2] / BZ);
-    // one persistent block per CU: the weights are staged in LDS once per block
-    const int blocks = bricks < cu_count() ? bricks : cu_count();
-    hipLaunchKernelGGL((k_conv<CIN, COUT>), dim3(blocks), dim3(256), 0, st, a, bricks);
+    // one persistent block per CU: the weights are staged in LDS once per block, and its two
+    // halves work on two bricks at a time
+    const int pairs = (bricks + 1) / 2;
+    const int blocks = pairs < cu_count() ? pairs : cu_count();
+    hipLaunchKernelGGL((k_conv<CIN, COUT>), dim3(blocks), dim3(512), 0, st, a, bricks);
     NOPCK(hipGetLastError());
     return 0;
 }

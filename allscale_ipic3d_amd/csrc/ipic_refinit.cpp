@@ -229,7 +229,9 @@ int ipic_ref_cell_counts(const ipic_desc* desc, const ipic_ref_distribution* dis
     const uint32_t pos_seed = dist->seed + 1u;  // uniform_pos_normal_speed: posGen seed + 1
     std::vector<uint64_t> hist((size_t)g.ncell, 0);
     // parallel part: chunks of pseudo particles from jumped-ahead LCG states
-    const int T = thread_count(dist->nthreads, (int64_t)(npseudo / 4096 + 1));
+    // one histogram of 4-byte counts per thread: at most 1 GiB of them in all
+    const int T = std::max<int>(1, std::min<int64_t>(thread_count(dist->nthreads, (int64_t)(npseudo / 4096 + 1)),
+                                                     (int64_t)(1 << 28) / g.ncell));
     std::vector<std::vector<uint32_t>> th(T);
     std::vector<uint64_t> first_redraw(T, npseudo);
     std::vector<std::thread> pool;
