@@ -32,7 +32,15 @@
 namespace {
 
 typedef short v4s __attribute__((ext_vector_type(4)));
+typedef short v8s __attribute__((ext_vector_type(8)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 typedef float v4f __attribute__((ext_vector_type(4)));
+
+// 8 consecutive BF16 values from LDS at an 8-byte aligned address, as an MFMA operand
+__device__ __forceinline__ v8bf lds8(const unsigned char* p) {
+    const v4s lo = *(const v4s*)p, hi = *(const v4s*)(p + 8);
+    return __builtin_bit_cast(v8bf, __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7));
+}
 
 constexpr int BX = 4, BY = 4, BZ = 16;               // output brick of a block
 constexpr int BV = BX * BY * BZ;                     // 256 voxels
@@ -156,19 +164,39 @@ __global__ __launch_bounds__(512) void k_conv(ConvArgs a, int nbricks) {
 #pragma unroll
                 for (int dz = 0; dz < 3; ++dz) {
                     const int tap = (dx * 3 + dy) * 3 + dz;
+                    if constexpr (CIN % 32 == 0 && COUT >= 32) {
+                        // 32 input channels per MFMA (v_mfma_f32_16x16x32_bf16): lane (m, kq)
+                        // holds channels 8 kq .. 8 kq + 7 of its row / column
 #pragma unroll
-                    for (int s2 = 0; s2 < CIN / 16; ++s2) {
-                        v4s bf[NT];
-#pragma unroll
-                        for (int t = 0; t < NT; ++t)
-                            bf[t] = *(const v4s*)(sw + (tap * COUT + t * 16 + m) * SV + (s2 * 16 + kq * 4) * 2);
-#pragma unroll
-                        for (int y = 0; y < BY; ++y) {
-                            const int hv = ((w + dx) * HY + (y + dy)) * HZ + (m + dz);
-                            const v4s af = *(const v4s*)(hx_ + hv * SV + (s2 * 16 + kq * 4) * 2);
+                        for (int s2 = 0; s2 < CIN / 32; ++s2) {
+                            v8bf bf[NT];
 #pragma unroll
                             for (int t = 0; t < NT; ++t)
-                                acc[y][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af, bf[t], acc[y][t], 0, 0, 0);
+                                bf[t] = lds8(sw + (tap * COUT + t * 16 + m) * SV + (s2 * 32 + kq * 8) * 2);
+#pragma unroll
+                            for (int y = 0; y < BY; ++y) {
+                                const int hv = ((w + dx) * HY + (y + dy)) * HZ + (m + dz);
+                                const v8bf af = lds8(hx_ + hv * SV + (s2 * 32 + kq * 8) * 2);
+#pragma unroll
+                                for (int t = 0; t < NT; ++t)
+                                    acc[y][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, bf[t], acc[y][t], 0, 0, 0);
+                            }
+                        }
+                    } else {
+#pragma unroll
+                        for (int s2 = 0; s2 < CIN / 16; ++s2) {
+                            v4s bf[NT];
+#pragma unroll
+                            for (int t = 0; t < NT; ++t)
+                                bf[t] = *(const v4s*)(sw + (tap * COUT + t * 16 + m) * SV + (s2 * 16 + kq * 4) * 2);
+#pragma unroll
+                            for (int y = 0; y < BY; ++y) {
+                                const int hv = ((w + dx) * HY + (y + dy)) * HZ + (m + dz);
+                                const v4s af = *(const v4s*)(hx_ + hv * SV + (s2 * 16 + kq * 4) * 2);
+#pragma unroll
+                                for (int t = 0; t < NT; ++t)
+                                    acc[y][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af, bf[t], acc[y][t], 0, 0, 0);
+                            }
                         }
                     }
                 }
