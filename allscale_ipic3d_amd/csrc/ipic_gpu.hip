@@ -23,6 +23,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <exception>
 #include <vector>
 
 #include "ipic_gpu.h"
@@ -2331,7 +2332,13 @@ int ipic_gpu_init_reference(ipic_ctx* ctx, int32_t species, const ipic_ref_distr
     int64_t total = 0;
     for (int64_t c = 0; c < nc; ++c) total += cnt[(size_t)c];
     if (total >= (1LL << 31)) return fail(ctx, IPIC_E_RANGE, "at most 2^31-1 particles per species");
-    std::vector<double> pos((size_t)total * 3), vel((size_t)total * 3);
+    std::vector<double> pos, vel;
+    try {
+        pos.resize((size_t)total * 3);
+        vel.resize((size_t)total * 3);
+    } catch (const std::exception&) {
+        return fail(ctx, IPIC_E_RANGE, "host memory for the reference initialisation");
+    }
     if (int rc = ipic_ref_particles(&ctx->d, dist, cnt.data(), pos.data(), vel.data()))
         return fail(ctx, rc, ipic_ref_last_error());
     if (counts) std::memcpy(counts, cnt.data(), (size_t)nc * sizeof(int64_t));

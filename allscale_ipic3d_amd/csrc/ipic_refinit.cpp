@@ -32,6 +32,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <exception>
 #include <string>
 #include <thread>
 #include <vector>
@@ -219,7 +220,7 @@ extern "C" {
 
 const char* ipic_ref_last_error(void) { return g_ref_error.c_str(); }
 
-int ipic_ref_cell_counts(const ipic_desc* desc, const ipic_ref_distribution* dist, int64_t* counts) {
+static int cell_counts_impl(const ipic_desc* desc, const ipic_ref_distribution* dist, int64_t* counts) {
     Geometry g;
     if (int rc = geometry(desc, g)) return rc;
     if (int rc = check_dist(dist)) return rc;
@@ -289,8 +290,8 @@ int ipic_ref_cell_counts(const ipic_desc* desc, const ipic_ref_distribution* dis
     return 0;
 }
 
-int ipic_ref_particles(const ipic_desc* desc, const ipic_ref_distribution* dist, const int64_t* counts, double* pos,
-                       double* vel) {
+static int particles_impl(const ipic_desc* desc, const ipic_ref_distribution* dist, const int64_t* counts,
+                          double* pos, double* vel) {
     Geometry g;
     if (int rc = geometry(desc, g)) return rc;
     if (int rc = check_dist(dist)) return rc;
@@ -330,6 +331,24 @@ int ipic_ref_particles(const ipic_desc* desc, const ipic_ref_distribution* dist,
     }
     for (auto& p : pool) p.join();
     return 0;
+}
+
+// no C++ exception crosses the C ABI: a failed allocation or thread start is an error code
+int ipic_ref_cell_counts(const ipic_desc* desc, const ipic_ref_distribution* dist, int64_t* counts) {
+    try {
+        return cell_counts_impl(desc, dist, counts);
+    } catch (const std::exception& e) {
+        return ref_fail(IPIC_E_RANGE, std::string("reference init: ") + e.what());
+    }
+}
+
+int ipic_ref_particles(const ipic_desc* desc, const ipic_ref_distribution* dist, const int64_t* counts, double* pos,
+                       double* vel) {
+    try {
+        return particles_impl(desc, dist, counts, pos, vel);
+    } catch (const std::exception& e) {
+        return ref_fail(IPIC_E_RANGE, std::string("reference init: ") + e.what());
+    }
 }
 
 }  // extern "C"
