@@ -2014,6 +2014,10 @@ inline int host_cid(const Geo& g, int x, int y, int z) {
 
 int alloc_species_buffers(ipic_ctx* ctx, int s, long long cap) {
     Species& S = ctx->sp[s];
+    // the six arrays of a species are addressed through one buffer resource (< 4 GiB); checked
+    // before anything is freed, so a refused size leaves the species as it was
+    if (std::max<long long>(cap, 1) * 24 >= (1LL << 32))
+        return fail(ctx, IPIC_E_RANGE, "a species holds at most 178956970 particles");
     if (cap > S.cap) {
         for (int b = 0; b < 2; ++b) {
             dfree(ctx, S.X[b]);
@@ -2021,10 +2025,8 @@ int alloc_species_buffers(ipic_ctx* ctx, int s, long long cap) {
             S.X[b] = nullptr;
             S.ids[b] = nullptr;
         }
+        S.cap = 0;  // no buffers until both are allocated
         const long long c = std::max<long long>(cap, 1);
-        // the six arrays of a species are addressed through one buffer resource (< 4 GiB)
-        if (c * 24 >= (1LL << 32))
-            return fail(ctx, IPIC_E_RANGE, "a species holds at most 178956970 particles");
         for (int b = 0; b < 2; ++b) {
             if (int rc = dalloc(ctx, &S.X[b], (size_t)6 * c)) return rc;
             if (ctx->track)

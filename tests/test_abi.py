@@ -46,3 +46,40 @@ def test_desc_layout_matches_header():
     # ipic_field_operator: pointer, 5 floats, 2 int32, 3 doubles (4 bytes of padding)
     assert ctypes.sizeof(G.FieldOperatorDesc) == 8 + 20 + 8 + 4 + 24
     assert ctypes.sizeof(G.Conservation) == 9 * 8
+
+
+def _create(**kw):
+    """ipic_gpu_create with a valid 4^3 desc modified by kw; returns (rc, last_error)."""
+    import ctypes as C
+    L = G.load_library()
+    d = G.Desc()
+    d.n[:] = kw.get("n", (4, 4, 4))
+    d.cell_width[:] = kw.get("w", (1.0, 1.0, 1.0))
+    d.dt, d.c = kw.get("dt", 0.1), kw.get("c", 1.0)
+    d.nspecies = kw.get("nspecies", 1)
+    d.species[0].q, d.species[0].qom = -1.0, -1.0
+    d.field_substeps = kw.get("field_substeps", 1)
+    d.tile[:] = kw.get("tile", (0, 0, 0))
+    h = C.c_void_p()
+    rc = L.ipic_gpu_create(C.byref(d), C.byref(h))
+    if rc == 0:
+        L.ipic_gpu_destroy(h)
+    return rc, L.ipic_gpu_last_error(None).decode()
+
+
+@pytest.mark.parametrize("kw,code,msg", [
+    ({"n": (0, 4, 4)}, -1, "grid size"),
+    ({"w": (1.0, -1.0, 1.0)}, -1, "cell width"),
+    ({"dt": 0.0}, -1, "dt and c"),
+    ({"c": -1.0}, -1, "dt and c"),
+    ({"nspecies": 0}, -1, "nspecies"),
+    ({"nspecies": 9}, -1, "nspecies"),
+    ({"n": (2048, 1024, 1024)}, -4, "grid too large"),
+    ({"field_substeps": -1}, -1, "field_substeps"),
+    ({"tile": (3, 0, 0)}, -1, "tile size"),
+    ({"tile": (16, 0, 0), "n": (32, 4, 4)}, -1, "tile size"),
+])
+def test_create_rejects_invalid_descriptions(kw, code, msg):
+    """Argument checks of ipic_gpu_create run before any device call (no GPU needed)."""
+    rc, err = _create(**kw)
+    assert rc == code and msg in err, (rc, err)
