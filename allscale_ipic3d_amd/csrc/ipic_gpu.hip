@@ -36,14 +36,14 @@ namespace {
 #define IPIC_PUSH_THREADS 256    // threads per block of the particle kernel
 #endif
 #ifndef IPIC_KRUN
-#define IPIC_KRUN 5              // max particles per run (one lane, held in registers)
+#define IPIC_KRUN 6              // max particles per run (one lane, held in registers)
 #endif
 #ifndef IPIC_JHALO
 #define IPIC_JHALO 0             // node layers of the tile J accumulator beyond the tile (0 or 1);
                                  // deposits beyond it go to the global overflow accumulator
 #endif
 #ifndef IPIC_PUSH_WAVES_PER_EU
-#define IPIC_PUSH_WAVES_PER_EU 3 // register budget of the particle kernel (0: compiler's choice)
+#define IPIC_PUSH_WAVES_PER_EU 2 // register budget of the particle kernel (0: compiler's choice)
 #endif
 
 constexpr int NT = 256;          // threads per block of the scan kernels
@@ -533,6 +533,20 @@ __device__ __forceinline__ void gather_lds(const float4* F4, const float2* F2, i
             }
 }
 
+// The same interpolation from the 8 corner values held in registers (corner (i,j,k) at
+// index (i*2+j)*2+k).
+__device__ __forceinline__ void gather_regs(const float4 n4[8], const float2 n2[8], float x, float y, float z,
+                                            float& Ex, float& Ey, float& Ez, float& Bx, float& By, float& Bz) {
+    float wt[8];
+    tri_weights(x, y, z, wt);
+    Ex = Ey = Ez = Bx = By = Bz = 0.0f;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        Ex = fmaf(n4[q].x, wt[q], Ex); Ey = fmaf(n4[q].y, wt[q], Ey); Ez = fmaf(n4[q].z, wt[q], Ez);
+        Bx = fmaf(n4[q].w, wt[q], Bx); By = fmaf(n2[q].x, wt[q], By); Bz = fmaf(n2[q].y, wt[q], Bz);
+    }
+}
+
 // Dynamic LDS layout of k_particles (bytes, 16-aligned sections)
 struct PLayout {
     int off_f4, off_f2, off_seg, off_cnt, off_stay, off_wlo, off_wave, wave_bytes, total;
@@ -728,6 +742,22 @@ __global__ __launch_bounds__(NTP) IPIC_PUSH_ATTR void k_particles(PushArgs A) {
             float mom[3][8];
 #pragma unroll
             for (int m = 0; m < 8; ++m) mom[0][m] = mom[1][m] = mom[2][m] = 0.0f;
+            // the run cell's 8 corner nodes, read from LDS once per run and held in registers
+            // for all the run's particles
+            float4 rn4[8];
+            float2 rn2[8];
+            if (MODE == MODE_PUSH && !DIPOLE) {
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+#pragma unroll
+                    for (int j = 0; j < 2; ++j)
+#pragma unroll
+                        for (int kk = 0; kk < 2; ++kk) {
+                            const int nn = nb + i * fsx + j * fsy + kk;
+                            rn4[(i * 2 + j) * 2 + kk] = sF4[nn];
+                            rn2[(i * 2 + j) * 2 + kk] = sF2[nn];
+                        }
+            }
             int run_in = 0, rank = 0, qn = 0, kn = 0;
             const int a_start = (MODE != MODE_DEPOSIT && active) ? S.start[cbase + c] : 0;
             const int a_end = (MODE != MODE_DEPOSIT && active) ? S.start[cbase + c + 1] : 0;
@@ -751,7 +781,7 @@ __global__ __launch_bounds__(NTP) IPIC_PUSH_ATTR void k_particles(PushArgs A) {
                                 Ex = Ey = Ez = 0.0f;
                                 dipole_field(g, ax, ay, az, px, py, pz, Bx, By, Bz);
                             } else {
-                                gather_lds(sF4, sF2, nb, fsx, fsy, px, py, pz, Ex, Ey, Ez, Bx, By, Bz);
+                                gather_regs(rn4, rn2, px, py, pz, Ex, Ey, Ez, Bx, By, Bz);
                             }
                             boris_move(g, qom, subk, Ex, Ey, Ez, Bx, By, Bz, vx, vy, vz, px, py, pz);
                             if (REFLECT) reflect_faces(g, ax, ay, az, px, py, pz, vx, vy, vz);
