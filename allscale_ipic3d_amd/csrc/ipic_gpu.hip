@@ -65,6 +65,7 @@ struct Geo {
     int NTl[3];
     int TC;
     int ntiles;
+    int nsl;              // blocks per tile of the particle kernel (slices of the tile's cells)
     int ncell;
     int NJ;
     int nj[3];            // J accumulator nodes per axis: T + 1 + 2 jh
@@ -88,7 +89,8 @@ struct Geo {
     float B0[3];
     float inv_T[3];       // 1 / T (exact small-integer division by a float reciprocal)
     float inv_T12;        // 1 / (T[1] * T[2])
-    float fix_pre;        // 2^(fix_bits - 32): J in fixed point with fix_bits fractional bits
+    float fix_lim;        // 2^(50 - fix_bits): J in fixed point with fix_bits fractional bits
+    double fix_scale;     // 2^fix_bits
     double fix_inv;       // 2^-fix_bits
 };
 
@@ -192,16 +194,15 @@ __device__ __forceinline__ int block_exscan(int v, int* s_w, int& total) {
 
 // J is accumulated in LDS as 64-bit fixed point (fix_bits fractional bits): integer atomics
 // are fast where float atomics on shared addresses serialise, and the sum is exact and
-// independent of the order of the adds.  With a = v * 2^(fix_bits - 32) (clamped to 2^30),
-// v * 2^fix_bits = hi * 2^32 + lo where hi = floor(a) is the signed high word and
-// lo = (a - hi) * 2^32, rounded to nearest, the unsigned low word: the two words are the
-// two's-complement 64-bit integer.  a - hi is exact except for -1 < a < 0, where it is
-// rounded to nearest; both roundings are unbiased, so sums over many particles keep no drift.
-__device__ __forceinline__ void fix_add(long long* p, float v, float pre) {
-    const float a = __builtin_amdgcn_fmed3f(v * pre, -1073741824.0f, 1073741824.0f);
-    const float h = floorf(a);
-    const unsigned lo = (unsigned)rintf(fminf((a - h) * 4294967296.0f, 4294967040.0f));
-    const long long q = (long long)(((unsigned long long)(unsigned)(int)h << 32) | lo);
+// independent of the order of the adds.  The conversion rounds v * 2^fix_bits to the nearest
+// integer x in one FP64 fused multiply-add with 1.5 * 2^52, whose unit in the last place is 1:
+// for |x| < 2^51 the bit pattern of the result is bits(1.5 * 2^52) + x, and as the constant's
+// low word is zero, x is that pattern with the constant's high word subtracted.  v is clamped
+// to |x| <= 2^50 first (NaN to the lower bound), so the identity always holds.
+__device__ __forceinline__ void fix_add(long long* p, float v, float lim, double scale) {
+    const float vc = fminf(fmaxf(v, -lim), lim);
+    const double d = fma((double)vc, scale, 6755399441055744.0);
+    const long long q = (long long)((unsigned long long)__double_as_longlong(d) - 0x4338000000000000ull);
     __hip_atomic_fetch_add(p, q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
@@ -270,9 +271,9 @@ __device__ __forceinline__ void deposit_direct(const PushArgs& A, const TG& G, l
                 const int nx = elx + JH + i, ny = ely + JH + j, nz = elz + JH + k;
                 if (nx >= 0 && nx < G.nj0 && ny >= 0 && ny < G.nj1 && nz >= 0 && nz < G.nj2) {
                     const int l = (nx * G.nj1 + ny) * G.nj2 + nz;
-                    fix_add(&sJ[l], jx * ww, g.fix_pre);
-                    fix_add(&sJ[G.NJ + l], jy * ww, g.fix_pre);
-                    fix_add(&sJ[2 * G.NJ + l], jz * ww, g.fix_pre);
+                    fix_add(&sJ[l], jx * ww, g.fix_lim, g.fix_scale);
+                    fix_add(&sJ[G.NJ + l], jy * ww, g.fix_lim, g.fix_scale);
+                    fix_add(&sJ[2 * G.NJ + l], jz * ww, g.fix_lim, g.fix_scale);
                 } else {
                     const int gid = nid_of(g, wrapc(x0 + elx + i, g.n[0]), wrapc(y0 + ely + j, g.n[1]),
                                            wrapc(z0 + elz + k, g.n[2]));
@@ -607,6 +608,7 @@ __global__ __launch_bounds__(NTP) IPIC_PUSH_ATTR void k_particles(PushArgs A) {
     float* K0 = (float*)(wbase + (((WS + 1) * 4 + WCH * 2 + 15) & ~15));  // leaver buffer
 
     const int t = A.tile_order ? A.tile_order[blockIdx.x] : xcd_tile(blockIdx.x, g.ntiles);
+    const int sl = blockIdx.y;  // this block's slice of the tile's cells
     const int tx = t / (g.NTl[1] * g.NTl[2]), ty = (t / g.NTl[2]) % g.NTl[1], tz = t % g.NTl[2];
     const int x0 = tx * G.T0, y0 = ty * G.T1, z0 = tz * G.T2;
     const int cbase = t * G.TC;
@@ -640,16 +642,23 @@ __global__ __launch_bounds__(NTP) IPIC_PUSH_ATTR void k_particles(PushArgs A) {
             s_stay[i] = 0;
         }
         __syncthreads();
-        // cells of wave w: [s_wlo[w], s_wlo[w+1]), split at equal particle counts
+        // the block's cells [clo, chi): slice sl of g.nsl at equal particle counts; cells of
+        // wave w: [s_wlo[w], s_wlo[w+1]), the slice split again at equal particle counts
         if (tid <= NW) {
+            auto first_ge = [&](long long target) {  // first cell c with s_seg[c] >= target
+                int lo = 0, hi = G.TC;
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (s_seg[mid] >= target) hi = mid; else lo = mid + 1;
+                }
+                return lo;
+            };
             const long long tot = (long long)s_seg[G.TC] - s_seg[0];
-            const int target = s_seg[0] + (int)(tot * tid / NW);
-            int lo = 0, hi = G.TC;  // first cell c with s_seg[c] >= target
-            while (lo < hi) {
-                const int mid = (lo + hi) >> 1;
-                if (s_seg[mid] >= target) hi = mid; else lo = mid + 1;
-            }
-            s_wlo[tid] = (tid == 0) ? 0 : (tid == NW ? G.TC : lo);
+            const int clo = (sl == 0) ? 0 : first_ge(s_seg[0] + tot * sl / g.nsl);
+            const int chi = (sl == g.nsl - 1) ? G.TC : first_ge(s_seg[0] + tot * (sl + 1) / g.nsl);
+            const long long sub = (long long)s_seg[chi] - s_seg[clo];
+            s_wlo[tid] = (tid == 0) ? clo
+                       : (tid == NW ? chi : max(clo, min(chi, first_ge(s_seg[clo] + sub * tid / NW))));
         }
         __syncthreads();
         const int chi = s_wlo[wid + 1];
@@ -760,7 +769,8 @@ __global__ __launch_bounds__(NTP) IPIC_PUSH_ATTR void k_particles(PushArgs A) {
             }
             int run_in = 0, rank = 0, qn = 0, kn = 0;
             const int a_start = (MODE != MODE_DEPOSIT && active) ? S.start[cbase + c] : 0;
-            const int a_end = (MODE != MODE_DEPOSIT && active) ? S.start[cbase + c + 1] : 0;
+            // slots of the run cell's stayers lie in [0, a_lim): below its end and the capacity
+            const int a_lim = (MODE != MODE_DEPOSIT && active) ? (int)min((long long)S.start[cbase + c + 1], S.cap) : 0;
 
             // pass B, fast path
 #pragma unroll
@@ -797,7 +807,7 @@ __global__ __launch_bounds__(NTP) IPIC_PUSH_ATTR void k_particles(PushArgs A) {
                         if (MODE != MODE_DEPOSIT) {
                             const int oidx = a_start + base + rank;
                             ++rank;
-                            if (oidx < 0 || oidx >= a_end || (long long)oidx >= S.cap) {
+                            if ((unsigned)oidx >= (unsigned)a_lim) {
                                 atomicOr(A.err, 1);
                             } else {
                                 store_particle(rout, cs, oidx, sunk ? NaNf : px, py, pz, vx, vy, vz);
@@ -859,7 +869,7 @@ __global__ __launch_bounds__(NTP) IPIC_PUSH_ATTR void k_particles(PushArgs A) {
 #pragma unroll
                             for (int kk = 0; kk < 2; ++kk) {
                                 const int l = ((lx + JH + i) * G.nj1 + (ly + JH + j)) * G.nj2 + (lz + JH + kk);
-                                fix_add(&sJ[d * G.NJ + l], N[(i * 2 + j) * 2 + kk], g.fix_pre);
+                                fix_add(&sJ[d * G.NJ + l], N[(i * 2 + j) * 2 + kk], g.fix_lim, g.fix_scale);
                             }
                 }
             }
@@ -977,7 +987,7 @@ __global__ __launch_bounds__(NTP) IPIC_PUSH_ATTR void k_particles(PushArgs A) {
     }
     __syncthreads();
     if (MODE != MODE_REBIN) {
-        float* dst = A.jpart + (long long)t * 3 * G.NJ;
+        float* dst = A.jpart + ((long long)t * g.nsl + sl) * 3 * G.NJ;
         for (int i = tid; i < 3 * G.NJ; i += NTP) dst[i] = (float)((double)sJ[i] * g.fix_inv);
     }
 }
@@ -1020,7 +1030,7 @@ __device__ __forceinline__ void xyz_of(const Geo& g, int i, int& x, int& y, int&
 }
 
 // The cross-tile moment sum: J of node (x,y,z) summed in FP64 over every tile partial covering
-// it plus the overflow accumulator (cleared here), rounded once to FP32.
+// it (every slice of the tile) plus the overflow accumulator (cleared here), rounded once to FP32.
 __device__ __forceinline__ float4 sum_partials(const FieldArgs& F, int i, int x, int y, int z) {
     const Geo& g = F.g;
     double sx = 0.0, sy = 0.0, sz = 0.0;
@@ -1035,10 +1045,12 @@ __device__ __forceinline__ float4 sum_partials(const FieldArgs& F, int i, int x,
                 const int tc = pz[1 + cc] >> 8, lc = pz[1 + cc] & 255;
                 const int tile = (ta * g.NTl[1] + tb) * g.NTl[2] + tc;
                 const int l = (la * g.nj[1] + lb) * g.nj[2] + lc;
-                const float* base = F.jpart + (long long)tile * 3 * g.NJ;
-                sx += (double)base[l];
-                sy += (double)base[g.NJ + l];
-                sz += (double)base[2 * g.NJ + l];
+                const float* base = F.jpart + (long long)tile * g.nsl * 3 * g.NJ;
+                for (int s = 0; s < g.nsl; ++s, base += 3 * g.NJ) {
+                    sx += (double)base[l];
+                    sy += (double)base[g.NJ + l];
+                    sz += (double)base[2 * g.NJ + l];
+                }
             }
     if (*F.jflag) {
         sx += (double)F.jover[3 * i + 0];
@@ -1778,7 +1790,7 @@ int launch_particles(ipic_ctx* ctx, int mode) {
     }
     PushArgs A = push_args(ctx);
     const Geo& g = ctx->g;
-    dim3 grid(g.ntiles), block(NTP);
+    dim3 grid(g.ntiles, g.nsl), block(NTP);
     const bool dip = (ctx->d.flags & IPIC_FLAG_DIPOLE_MOVER) != 0;
     const size_t lds =
         (size_t)particle_layout(g.NJ, g.TC, (mode == MODE_PUSH && !dip) ? field_nodes(g) : 0).total;
@@ -2192,6 +2204,17 @@ int ipic_gpu_create(const ipic_desc* desc, ipic_ctx** out) {
     }
     g.TC = g.T[0] * g.T[1] * g.T[2];
     g.ntiles = g.NTl[0] * g.NTl[1] * g.NTl[2];
+    {
+        // One block per tile leaves most CUs idle on a grid of few tiles (8^3 cells: one tile).
+        // The particle kernel then splits each tile's cells among several blocks, up to two
+        // blocks per CU and at least as many cells per block as it has waves.
+        int cus = 256, dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+            cus = 256;
+        const int want = (2 * cus + g.ntiles - 1) / g.ntiles;
+        g.nsl = std::max(1, std::min(want, g.TC / NW));
+    }
     g.ncell = (int)ncell;
     g.NJ = g.nj[0] * g.nj[1] * g.nj[2];
     g.nmax = std::max(g.n[0], std::max(g.n[1], g.n[2]));
@@ -2210,7 +2233,8 @@ int ipic_gpu_create(const ipic_desc* desc, ipic_ctx** out) {
         int bits = 40;
         if (qmax * d.c > 0.0 && std::isfinite(qmax * d.c)) bits = 30 - (int)std::ceil(std::log2(qmax * d.c));
         bits = std::max(-60, std::min(120, bits));
-        g.fix_pre = (float)std::ldexp(1.0, bits - 32);
+        g.fix_lim = (float)std::ldexp(1.0, 50 - bits);
+        g.fix_scale = std::ldexp(1.0, bits);
         g.fix_inv = std::ldexp(1.0, -bits);
     }
     g.sink = d.sink ? 1 : 0;
@@ -2241,7 +2265,7 @@ int ipic_gpu_create(const ipic_desc* desc, ipic_ctx** out) {
     rc = rc ? rc : dalloc(ctx, &ctx->Bc, nc);
     rc = rc ? rc : dalloc(ctx, &ctx->Bn, nc);
     rc = rc ? rc : dalloc(ctx, &ctx->J, nc);
-    rc = rc ? rc : dalloc(ctx, &ctx->jpart, (size_t)g.ntiles * 3 * g.NJ);
+    rc = rc ? rc : dalloc(ctx, &ctx->jpart, (size_t)g.ntiles * g.nsl * 3 * g.NJ);
     rc = rc ? rc : dalloc(ctx, &ctx->jover, nc * 3);
     rc = rc ? rc : dalloc(ctx, &ctx->jflag, 1);
     rc = rc ? rc : dalloc(ctx, &ctx->jpairs, jp.size());

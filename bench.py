@@ -171,6 +171,60 @@ def surrogate_leg(G, cells, ppc, phases=10, seed=21):
     }
 
 
+def other_configs(G):
+    """Per-step times of the other configurations (parity-test cases, not the metric): the
+    two-species 64^3 x 64 ppc box, the uneven-load 128^3 dipole configuration (30-299 ppc,
+    74.9M particles, sink) and the reference main driver's inputs/test.inp and tiny.inp cases
+    (dipole mover, reflection, sink, the reference's random streams)."""
+    res = {}
+    sim = make_sim(G, 64, 64, 5)
+    sim.step(3)
+    st = sim.step(20)
+    t = (st["first_step_s"] + st["remaining_steps_s"]) / 20
+    res["two_species_64c_64ppc"] = {"particles": 64 ** 3 * 64, "ms_per_step": t * 1e3,
+                                     "updates_per_s": 64 ** 3 * 64 / t}
+    sim.close()
+
+    n = 128
+    c = n / 2.0
+    x = np.arange(n) + 0.5
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    d = np.stack([X - c, Y - c, Z - c], axis=-1).reshape(-1, 3)
+    r2 = np.sum(d * d, axis=1)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        fac = 1.0 / r2 ** 2.5
+    B = np.stack([3 * d[:, 0] * d[:, 2] * fac, 3 * d[:, 1] * d[:, 2] * fac,
+                  (2 * d[:, 2] ** 2 - d[:, 0] ** 2 - d[:, 1] ** 2) * fac], axis=1)
+    B[r2 <= 16.0] = 0.0
+    ppc = np.rint(30 * (1 + 9 * np.exp(-r2 / 400.0))).astype(np.int32)
+    q = -1.0 / (30 * FOUR_PI)
+    sim = G.Simulation(n, dt=0.1, species=((q, -1.0),), sink_center=(c, c, c), sink_radius=4.0)
+    sim.init_particles(0, ppc, (0.05,) * 3, drift=(0.1, 0.0, 0.0), seed=3)
+    sim.upload_fields(np.zeros((n ** 3, 3)), B)
+    sim.step(2)
+    st = sim.step(5)
+    t = (st["first_step_s"] + st["remaining_steps_s"]) / 5
+    res["uneven_dipole_128c"] = {"particles": int(ppc.sum()), "ppc_range": [int(ppc.min()), int(ppc.max())],
+                                 "ms_per_step": t * 1e3, "updates_per_s": float(ppc.sum()) / t}
+    sim.close()
+
+    from allscale_ipic3d_amd import driver as D
+    for name in ("test", "tiny"):
+        inp = os.path.join(ROOT, "tests", "golden", f"{name}_case.inp")
+        if not os.path.exists(inp):
+            continue
+        case = D.driver_case(D.Parameters(inp), name)
+        sim = D.make_simulation(case)
+        sim.init_reference(0, D.case_distribution(case))
+        sim.step(2)
+        st = sim.step(20)
+        t = (st["first_step_s"] + st["remaining_steps_s"]) / 20
+        res[f"reference_driver_{name}"] = {"cells": list(case.n), "particles": case.num_particles,
+                                           "ms_per_step": t * 1e3, "updates_per_s": case.num_particles / t}
+        sim.close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -185,6 +239,7 @@ def main():
     ap.add_argument("--no-copy-peak", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the learned field step leg")
     ap.add_argument("--surrogate-cells", type=int, default=64)
+    ap.add_argument("--no-other-configs", action="store_true", help="skip the per-step times of the other configurations")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy 10 times (rocprof counter calibration)")
     args = ap.parse_args()
@@ -245,6 +300,10 @@ def main():
     if not args.no_surrogate:
         sur = surrogate_leg(G, args.surrogate_cells, 64)
         log("surrogate:", {k: v for k, v in sur.items() if k != "training"})
+    others = None
+    if not args.no_other_configs:
+        others = other_configs(G)
+        log("other configurations:", others)
     cpu = None if args.no_cpu_baseline else cpu_baseline(cells, ppc)
     if cpu:
         log("cpu baseline:", cpu)
@@ -289,6 +348,7 @@ def main():
         },
         "cpu_baseline": cpu,
         "surrogate": sur,
+        "other_configs": others,
     }
     print(json.dumps(out))
 

@@ -116,6 +116,24 @@ def test_two_species_config2_shape_uniform_background_mode():
     _compare_particles(u, sim, [-1.0, 1.0 / 64], (8, 8, 16), (1, 1, 1), 0.05)
 
 
+@pytest.mark.parametrize("tile", [(0, 0, 0), (2, 2, 2)])
+def test_tile_slices_and_single_block_tiles(tile):
+    """16^3 cells x 16 particles per cell of each of two species, 6 steps.  With the default
+    8^3 tiles (8 tiles) the particle kernel splits every tile's cells among many blocks, whose
+    J partials the E pass sums; with 2^3 tiles (512 tiles) each tile is one block.  Both match
+    the oracle."""
+    case = make_case(n=(16, 16, 16), ppc=16, species=((-1.0, 0.05), (1.0 / 64, 0.006)), dt=0.05,
+                     B=(0, 0.2, 1.0), E_sigma=1e-3, seed=9)
+    u = case[0]
+    sim = gpu_sim_for(case, tile=tile)
+    u.step(6)
+    st = sim.step(6)
+    assert st["particles"] == 2 * 16 ** 3 * 16  # 16 per cell of each species
+    _compare_fields(u, sim)
+    _compare_particles(u, sim, [-1.0, 1.0 / 64], (16, 16, 16), (1, 1, 1), 0.05)
+    _counts_close(u, sim, 4)
+
+
 def test_fields_only_leapfrog():
     rng = np.random.default_rng(5)
     n = (6, 8, 10)
