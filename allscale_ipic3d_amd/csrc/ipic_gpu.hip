@@ -5,8 +5,8 @@
 //              units) and velocities vx,vy,vz; 24 B per particle; sorted by cell in tile-major
 //              cell order (cid = tile * TC + local cell), ping-pong buffers X[0], X[1].
 //   fields     E (nodes), Bc (centres), Bn (nodes), J (nodes) as float4 rows, row-major.
-//   J partials per tile: the tile's nodes plus a one-node halo, (T+3)^3 x 3 FP32, summed
-//              across tiles in FP64 by the E pass.
+//   J partials per tile and block slice: the tile's nodes plus IPIC_JHALO node layers on each
+//              side, (T+1+2h)^3 x 3 FP32, summed across tiles and slices in FP64 by the E pass.
 //
 // The particle array lags its sort key by one step: X_n is sorted by the cell each particle
 // occupied when the previous push began (its "segment" a) and stores offsets relative to a,
@@ -54,7 +54,7 @@ constexpr int KRUN = IPIC_KRUN;
 constexpr int WCH = WS * KRUN;   // max particles per wave chunk
 constexpr int K0CAP = 32;        // leaver buffer entries per wave chunk
 constexpr int TCMAX = 512;       // max cells per tile
-constexpr int NJMAX = 11 * 11 * 11;  // max J nodes per tile ((T+3)^3, T <= 8)
+constexpr int NJMAX = 11 * 11 * 11;  // max J nodes per tile ((T+1+2h)^3, T <= 8, h <= 1)
 constexpr int JPMAX = 4;         // max (tile, local) pairs per axis coordinate
 
 enum { MODE_PUSH = 0, MODE_DEPOSIT = 1, MODE_REBIN = 2 };
