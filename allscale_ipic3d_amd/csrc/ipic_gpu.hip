@@ -43,7 +43,8 @@ namespace {
                                  // deposits beyond it go to the global overflow accumulator
 #endif
 #ifndef IPIC_PUSH_WAVES_PER_EU
-#define IPIC_PUSH_WAVES_PER_EU 2 // register budget of the particle kernel (0: compiler's choice)
+#define IPIC_PUSH_WAVES_PER_EU 3 // register budget of the particle kernel, waves per SIMD (0: the
+                                 // compiler's choice); see push_waves()
 #endif
 
 constexpr int NT = 256;          // threads per block of the scan kernels
@@ -303,9 +304,11 @@ __device__ __forceinline__ void count_add(const Geo& g, const TG& G, int* s_arr,
 // count and every wave streams its cells on its own: it cuts them into runs of at most KRUN
 // particles of one cell, one run per lane, and each lane loads its run straight into registers
 // (all its loads in flight at once).  The fast path covers particles that start and end the
-// step in the run's cell: the gather evaluates per-run trilinear coefficients and the deposit
-// accumulates per-run moments, both in registers; the updated particle goes directly to its
-// slot of the output array.  Every other particle (it sits in another cell, or leaves the
+// step in the run's cell, in two passes over the run registers: the push (gather from the run
+// cell's 8 corner nodes, held in registers) updates them in place, then the deposit accumulates
+// per-run moments in registers and the updated particle goes directly to its slot of the output
+// array.  The corner nodes and the moments are never live together (168 VGPRs: 3 waves per
+// SIMD).  Every other particle (it sits in another cell, or leaves the
 // cell) is appended to a wave queue by ballot compaction and handled afterwards with all lanes
 // busy.  The tile's J accumulator (64-bit fixed point) and per-cell counters live in LDS.
 // ---------------------------------------------------------------------------------------------
@@ -574,14 +577,23 @@ __host__ __device__ inline PLayout particle_layout(int NJ, int TC, int NF) {
 
 __host__ __device__ inline int field_nodes(const Geo& g) { return (g.T[0] + 1) * (g.T[1] + 1) * (g.T[2] + 1); }
 
-#if IPIC_PUSH_WAVES_PER_EU > 0
-#define IPIC_PUSH_ATTR __attribute__((amdgpu_waves_per_eu(IPIC_PUSH_WAVES_PER_EU)))
+// option bits of the particle kernel
+enum { OPT_FIX8 = 1, OPT_SINK = 2, OPT_REFLECT = 4, OPT_DIPOLE = 8 };
+
+// Register budget (waves per SIMD) of a particle-kernel instance.  The plain push fits 168 VGPRs
+// (3 waves) by itself and runs 1-2 % faster without the bound; the sink and reflection checks
+// push it to 171 unless bounded; with particle ids it needs 171-177 and would spill under the
+// bound, so it keeps 2 waves.
+constexpr int push_waves(bool track, int opt) {
+    return track ? IPIC_PUSH_WAVES_PER_EU - 1
+                 : ((opt & (OPT_SINK | OPT_REFLECT)) ? IPIC_PUSH_WAVES_PER_EU : IPIC_PUSH_WAVES_PER_EU - 1);
+}
+
+#if IPIC_PUSH_WAVES_PER_EU > 1
+#define IPIC_PUSH_ATTR __attribute__((amdgpu_waves_per_eu(push_waves(TRACK, OPT))))
 #else
 #define IPIC_PUSH_ATTR
 #endif
-
-// option bits of the particle kernel
-enum { OPT_FIX8 = 1, OPT_SINK = 2, OPT_REFLECT = 4, OPT_DIPOLE = 8 };
 
 template <int MODE, bool TRACK, int OPT>
 __global__ __launch_bounds__(NTP) IPIC_PUSH_ATTR void k_particles(PushArgs A) {
@@ -736,101 +748,116 @@ __global__ __launch_bounds__(NTP) IPIC_PUSH_ATTR void k_particles(PushArgs A) {
             const int ax = x0 + lx, ay = y0 + ly, az = z0 + lz;
             const int nb = lx * fsx + ly * fsy + lz;  // the run cell's first node in the LDS field
 
-            // pass A: stayers of each run (particles of segment a still in cell a; NaN: no)
-            int my_stay = 0;
-            if (MODE != MODE_DEPOSIT) {
+            // pass A: the run's home particles, bit jj of hm (particles of segment a still in cell
+            // a, the fast path; NaN: no); they are the stayers that take the cell's first slots
+            unsigned hm = 0;
 #pragma unroll
-                for (int jj = 0; jj < KRUN; ++jj)
-                    if (in_cell(rx[jj], ry[jj], rz[jj])) ++my_stay;
-            }
+            for (int jj = 0; jj < KRUN; ++jj)
+                if (in_cell(rx[jj], ry[jj], rz[jj])) hm |= 1u << jj;
+            const int my_stay = (MODE != MODE_DEPOSIT) ? __popc(hm) : 0;
             const int sincl = wave_incl_scan(my_stay, lane);
             const int sex = sincl - my_stay;
             const int sfirst = __shfl(sex, R[k], WS);
             const int base = (k == 0 ? carry : 0) + sex - sfirst;
 
-            float mom[3][8];
-#pragma unroll
-            for (int m = 0; m < 8; ++m) mom[0][m] = mom[1][m] = mom[2][m] = 0.0f;
-            // the run cell's 8 corner nodes, read from LDS once per run and held in registers
-            // for all the run's particles
-            float4 rn4[8];
-            float2 rn2[8];
-            if (MODE == MODE_PUSH && !DIPOLE) {
-#pragma unroll
-                for (int i = 0; i < 2; ++i)
-#pragma unroll
-                    for (int j = 0; j < 2; ++j)
-#pragma unroll
-                        for (int kk = 0; kk < 2; ++kk) {
-                            const int nn = nb + i * fsx + j * fsy + kk;
-                            rn4[(i * 2 + j) * 2 + kk] = sF4[nn];
-                            rn2[(i * 2 + j) * 2 + kk] = sF2[nn];
-                        }
-            }
             int run_in = 0, rank = 0, qn = 0, kn = 0;
             const int a_start = (MODE != MODE_DEPOSIT && active) ? S.start[cbase + c] : 0;
             // slots of the run cell's stayers lie in [0, a_lim): below its end and the capacity
             const int a_lim = (MODE != MODE_DEPOSIT && active) ? (int)min((long long)S.start[cbase + c + 1], S.cap) : 0;
 
-            // pass B, fast path
+            // pass B1, push of the home particles in place in the run registers (gather, Boris,
+            // reflection, sink); a removed particle gets a NaN x offset.  The run cell's 8 corner
+            // nodes are read from LDS once per run and held in registers for this pass only, so
+            // they never share the register file with the deposit moments of pass B2.
+            if (MODE == MODE_PUSH) {
+                float4 rn4[8];
+                float2 rn2[8];
+                if (!DIPOLE) {
+#pragma unroll
+                    for (int i = 0; i < 2; ++i)
+#pragma unroll
+                        for (int j = 0; j < 2; ++j)
+#pragma unroll
+                            for (int kk = 0; kk < 2; ++kk) {
+                                const int nn = nb + i * fsx + j * fsy + kk;
+                                rn4[(i * 2 + j) * 2 + kk] = sF4[nn];
+                                rn2[(i * 2 + j) * 2 + kk] = sF2[nn];
+                            }
+                }
+#pragma unroll
+                for (int jj = 0; jj < KRUN; ++jj) {
+                    if (hm & (1u << jj)) {
+                        float px = rx[jj], py = ry[jj], pz = rz[jj];
+                        float vx = rvx[jj], vy = rvy[jj], vz = rvz[jj];
+                        float Ex, Ey, Ez, Bx, By, Bz;
+                        if (DIPOLE) {
+                            Ex = Ey = Ez = 0.0f;
+                            dipole_field(g, ax, ay, az, px, py, pz, Bx, By, Bz);
+                        } else {
+                            gather_regs(rn4, rn2, px, py, pz, Ex, Ey, Ez, Bx, By, Bz);
+                        }
+                        boris_move(g, qom, subk, Ex, Ey, Ez, Bx, By, Bz, vx, vy, vz, px, py, pz);
+                        if (REFLECT) reflect_faces(g, ax, ay, az, px, py, pz, vx, vy, vz);
+                        bool sunk = !finite3(px, py, pz);
+                        if (SINK && !sunk) {
+                            const int dex = dcell(px), dey = dcell(py), dez = dcell(pz);
+                            const float sx_ = g.origin[0] + ((float)wrapc(ax + dex, g.n[0]) + px - (float)dex) * g.w[0] - g.sink_c[0];
+                            const float sy_ = g.origin[1] + ((float)wrapc(ay + dey, g.n[1]) + py - (float)dey) * g.w[1] - g.sink_c[1];
+                            const float sz_ = g.origin[2] + ((float)wrapc(az + dez, g.n[2]) + pz - (float)dez) * g.w[2] - g.sink_c[2];
+                            sunk = (sx_ * sx_ + sy_ * sy_ + sz_ * sz_) <= g.sink_r2;
+                        }
+                        rx[jj] = sunk ? NaNf : px;
+                        ry[jj] = py;
+                        rz[jj] = pz;
+                        rvx[jj] = vx;
+                        rvy[jj] = vy;
+                        rvz[jj] = vz;
+                    }
+                }
+            }
+
+            // pass B2: slots, removal counts and deposit moments of the home particles; queue
+            // entries for the others (their registers are untouched by pass B1)
+            float mom[3][8];
+#pragma unroll
+            for (int m = 0; m < 8; ++m) mom[0][m] = mom[1][m] = mom[2][m] = 0.0f;
 #pragma unroll
             for (int jj = 0; jj < KRUN; ++jj) {
                 int qe = -1;       // arrival queue entry of this lane
                 bool leaver = false;
-                float px = rx[jj], py = ry[jj], pz = rz[jj];
-                float vx = rvx[jj], vy = rvy[jj], vz = rvz[jj];
-                const bool live = alive(px, py, pz);
-                const bool home = live && in_cell(px, py, pz);  // in the run's cell: fast path
-                if (live && !home) qe = (pst + jj - plo) | (k << 9);  // sits in another cell
-                {
-                    if (home) {
-                        bool sunk = false;
-                        if (MODE == MODE_PUSH) {
-                            float Ex, Ey, Ez, Bx, By, Bz;
-                            if (DIPOLE) {
-                                Ex = Ey = Ez = 0.0f;
-                                dipole_field(g, ax, ay, az, px, py, pz, Bx, By, Bz);
-                            } else {
-                                gather_regs(rn4, rn2, px, py, pz, Ex, Ey, Ez, Bx, By, Bz);
-                            }
-                            boris_move(g, qom, subk, Ex, Ey, Ez, Bx, By, Bz, vx, vy, vz, px, py, pz);
-                            if (REFLECT) reflect_faces(g, ax, ay, az, px, py, pz, vx, vy, vz);
-                            sunk = !finite3(px, py, pz);
-                            if (SINK && !sunk) {
-                                const int dex = dcell(px), dey = dcell(py), dez = dcell(pz);
-                                const float sx_ = g.origin[0] + ((float)wrapc(ax + dex, g.n[0]) + px - (float)dex) * g.w[0] - g.sink_c[0];
-                                const float sy_ = g.origin[1] + ((float)wrapc(ay + dey, g.n[1]) + py - (float)dey) * g.w[1] - g.sink_c[1];
-                                const float sz_ = g.origin[2] + ((float)wrapc(az + dez, g.n[2]) + pz - (float)dez) * g.w[2] - g.sink_c[2];
-                                sunk = (sx_ * sx_ + sy_ * sy_ + sz_ * sz_) <= g.sink_r2;
-                            }
+                const float px = rx[jj], py = ry[jj], pz = rz[jj];
+                const float vx = rvx[jj], vy = rvy[jj], vz = rvz[jj];
+                const bool home = (hm >> jj) & 1u;
+                if (!home && alive(px, py, pz)) qe = (pst + jj - plo) | (k << 9);  // sits in another cell
+                if (home) {
+                    // a home particle's x offset is NaN only when pass B1 removed it
+                    const bool sunk = (MODE == MODE_PUSH) && __builtin_isnan(px);
+                    if (MODE != MODE_DEPOSIT) {
+                        const int oidx = a_start + base + rank;
+                        ++rank;
+                        if ((unsigned)oidx >= (unsigned)a_lim) {
+                            atomicOr(A.err, 1);
+                        } else {
+                            store_particle(rout, cs, oidx, px, py, pz, vx, vy, vz);
+                            if (TRACK) S.id_out[oidx] = rid[jj];
                         }
-                        if (MODE != MODE_DEPOSIT) {
-                            const int oidx = a_start + base + rank;
-                            ++rank;
-                            if ((unsigned)oidx >= (unsigned)a_lim) {
-                                atomicOr(A.err, 1);
-                            } else {
-                                store_particle(rout, cs, oidx, sunk ? NaNf : px, py, pz, vx, vy, vz);
-                                if (TRACK) S.id_out[oidx] = rid[jj];
-                            }
-                        }
-                        if (MODE == MODE_PUSH && sunk) {
-                            atomicAdd(A.removed, 1ull);
-                        } else if (MODE != MODE_REBIN) {
-                            if (MODE == MODE_DEPOSIT || in_cell(px, py, pz)) {
-                                const float jx = qs * vx, jy = qs * vy, jz = qs * vz;
-                                const float xy = px * py, xz = px * pz, yz = py * pz, xyz = xy * pz;
-                                const float mono[8] = {1.0f, px, py, pz, xy, xz, yz, xyz};
+                    }
+                    if (MODE == MODE_PUSH && sunk) {
+                        atomicAdd(A.removed, 1ull);
+                    } else if (MODE != MODE_REBIN) {
+                        if (MODE == MODE_DEPOSIT || in_cell(px, py, pz)) {
+                            const float jx = qs * vx, jy = qs * vy, jz = qs * vz;
+                            const float xy = px * py, xz = px * pz, yz = py * pz, xyz = xy * pz;
+                            const float mono[8] = {1.0f, px, py, pz, xy, xz, yz, xyz};
 #pragma unroll
-                                for (int m = 0; m < 8; ++m) {
-                                    mom[0][m] = fmaf(jx, mono[m], mom[0][m]);
-                                    mom[1][m] = fmaf(jy, mono[m], mom[1][m]);
-                                    mom[2][m] = fmaf(jz, mono[m], mom[2][m]);
-                                }
-                                ++run_in;
-                            } else {
-                                leaver = true;  // leaves cell a: deposit and count after the loop
+                            for (int m = 0; m < 8; ++m) {
+                                mom[0][m] = fmaf(jx, mono[m], mom[0][m]);
+                                mom[1][m] = fmaf(jy, mono[m], mom[1][m]);
+                                mom[2][m] = fmaf(jz, mono[m], mom[2][m]);
                             }
+                            ++run_in;
+                        } else {
+                            leaver = true;  // leaves cell a: deposit and count after the loop
                         }
                     }
                 }

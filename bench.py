@@ -83,18 +83,25 @@ def cpu_baseline(cells: int, ppc: int, budget_s: float = 12.0):
     rng = np.random.default_rng(1)
     u.set_fields(rng.normal(0.0, 1e-3, size=(n ** 3, 3)), np.tile([0.0, 0.2, 1.0], (n ** 3, 1)))
     npart = u.num_particles()
+
+    def timed(nthreads, budget, max_steps):
+        steps, t0 = 0, time.perf_counter()
+        while True:
+            u.step(1, nthreads)
+            steps += 1
+            el = time.perf_counter() - t0
+            if el > budget or steps >= max_steps:
+                return steps, el
+
     u.step(1, threads)  # warm-up
-    steps, t0 = 0, time.perf_counter()
-    while True:
-        u.step(1, threads)
-        steps += 1
-        el = time.perf_counter() - t0
-        if el > budget_s or steps >= 40:
-            break
+    steps, el = timed(threads, budget_s, 40)
+    steps1, el1 = timed(1, budget_s / 2, 5)  # the same step on one core
     u.close()
     return {"value": npart * steps / el, "unit": "particle updates/s", "cores": threads, "kind": "port",
+            "value_1core": npart * steps1 / el1,
             "sample": f"FP64 oracle step (deposit, leapfrog, gather+Boris, rebin), {n}^3 cells x {ppc} ppc "
-                      f"({npart} particles), {steps} steps in {el:.1f} s, {threads} threads"}
+                      f"({npart} particles), {steps} steps in {el:.1f} s on {threads} threads; "
+                      f"{steps1} steps in {el1:.1f} s on 1 thread"}
 
 
 def copy_peak_gbs(G):

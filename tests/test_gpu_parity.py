@@ -237,6 +237,25 @@ def test_dense_ragged_cells_and_chunks():
     _compare_particles(u, sim, [-1.0], n, (1, 1, 1), 0.05)
 
 
+def test_one_crowded_cell_among_tile_slices():
+    """8^3 cells are one tile, split among many blocks at equal particle counts: with 3000
+    particles in one cell and one in every other cell, the crowded cell belongs to one block
+    (and one wave) while most slices hold a few particles or none; fast particles spread it out
+    over 4 steps."""
+    n = (8, 8, 8)
+    ppc = np.ones(512, dtype=np.int64)
+    ppc[(3 * 8 + 4) * 8 + 5] = 3000
+    case = make_case(n=n, ppc=1, ppc_per_cell=ppc, species=((-1.0, 0.3),), dt=0.2, B=(0.0, 0.0, 0.5),
+                     E_sigma=1e-3, seed=12)
+    u = case[0]
+    sim = gpu_sim_for(case)
+    u.step(4)
+    sim.step(4)
+    _compare_fields(u, sim)
+    _compare_particles(u, sim, [-1.0], n, (1, 1, 1), 0.3)
+    _counts_close(u, sim, 4)
+
+
 def test_gpu_generator_matches_oracle_generator():
     import allscale_ipic3d_amd as G
     n = (4, 6, 8)
