@@ -54,6 +54,7 @@ constexpr int NW = NTP / WS;     // waves per block of the particle kernel
 constexpr int KRUN = IPIC_KRUN;
 constexpr int WCH = WS * KRUN;   // max particles per wave chunk
 constexpr int K0CAP = 32;        // leaver buffer entries per wave chunk
+constexpr int QCAP = WCH + WS;   // arrival queue entries per wave (fewer than WS carried over + one chunk)
 constexpr int TCMAX = 512;       // max cells per tile
 constexpr int NJMAX = 11 * 11 * 11;  // max J nodes per tile ((T+1+2h)^3, T <= 8, h <= 1)
 constexpr int JPMAX = 4;         // max (tile, local) pairs per axis coordinate
@@ -567,8 +568,8 @@ __host__ __device__ inline PLayout particle_layout(int NJ, int TC, int NF) {
     L.off_wlo = o; o += (NW + 1) * 4;
     o = (o + 15) & ~15;
     L.off_wave = o;
-    // per wave: run table (WS+1 ints), arrival queue (WCH ushort), leaver buffer (K0CAP x 8 floats)
-    L.wave_bytes = ((WS + 1) * 4 + WCH * 2 + 15) & ~15;
+    // per wave: run table (WS+1 ints), arrival queue (QCAP ints), leaver buffer (K0CAP x 8 floats)
+    L.wave_bytes = ((WS + 1) * 4 + QCAP * 4 + 15) & ~15;
     L.wave_bytes += K0CAP * 8 * 4;
     o += NW * L.wave_bytes;
     L.total = o;
@@ -585,8 +586,7 @@ enum { OPT_FIX8 = 1, OPT_SINK = 2, OPT_REFLECT = 4, OPT_DIPOLE = 8 };
 // push it to 171 unless bounded; with particle ids it needs 171-177 and would spill under the
 // bound, so it keeps 2 waves.
 constexpr int push_waves(bool track, int opt) {
-    return track ? IPIC_PUSH_WAVES_PER_EU - 1
-                 : ((opt & (OPT_SINK | OPT_REFLECT)) ? IPIC_PUSH_WAVES_PER_EU : IPIC_PUSH_WAVES_PER_EU - 1);
+    return track ? IPIC_PUSH_WAVES_PER_EU - 1 : IPIC_PUSH_WAVES_PER_EU;
 }
 
 #if IPIC_PUSH_WAVES_PER_EU > 1
@@ -616,8 +616,8 @@ __global__ __launch_bounds__(NTP) IPIC_PUSH_ATTR void k_particles(PushArgs A) {
     const int tid = threadIdx.x, lane = tid & (WS - 1), wid = tid / WS;
     unsigned char* wbase = smem + L.off_wave + wid * L.wave_bytes;
     int* R = (int*)wbase;                                            // WS+1 ints
-    unsigned short* Q = (unsigned short*)(wbase + (WS + 1) * 4);     // arrival queue
-    float* K0 = (float*)(wbase + (((WS + 1) * 4 + WCH * 2 + 15) & ~15));  // leaver buffer
+    int* Q = (int*)(wbase + (WS + 1) * 4);                           // arrival queue
+    float* K0 = (float*)(wbase + (((WS + 1) * 4 + QCAP * 4 + 15) & ~15));  // leaver buffer
 
     const int t = A.tile_order ? A.tile_order[blockIdx.x] : xcd_tile(blockIdx.x, g.ntiles);
     const int sl = blockIdx.y;  // this block's slice of the tile's cells
@@ -678,6 +678,108 @@ __global__ __launch_bounds__(NTP) IPIC_PUSH_ATTR void k_particles(PushArgs A) {
         int pos_cur = s_seg[c_cur];
         int carry = 0;  // stayers already placed for cell c_cur (when it continues)
         const float qom = S.qom, qs = S.qs, subk = S.subk;
+
+        // A particle sitting in another cell b (an arrival of the lagged sort, input index inq):
+        // full processing by one lane.  Its input cell is the last tile cell whose segment
+        // starts at or below inq.
+        auto arrive = [&](int inq) {
+            int kc = 0, khi = G.TC - 1;
+            while (kc < khi) {
+                const int mid = (kc + khi + 1) >> 1;
+                if (s_seg[mid] <= inq) kc = mid; else khi = mid - 1;
+            }
+            int qlx, qly, qlz;
+            G.split(kc, qlx, qly, qlz);
+            const unsigned vq = (unsigned)inq * 4u;
+            const float ox = ldp(rin, vq, 0), oy = ldp(rin, vq, cs), oz = ldp(rin, vq, 2 * cs);
+            float vx = ldp(rin, vq, 3 * cs), vy = ldp(rin, vq, 4 * cs), vz = ldp(rin, vq, 5 * cs);
+            const int dbx = dcell(ox), dby = dcell(oy), dbz = dcell(oz);
+            float px = ox - (float)dbx, py = oy - (float)dby, pz = oz - (float)dbz;
+            const int blx = qlx + dbx, bly = qly + dby, blz = qlz + dbz;  // tile-local cell b
+            const int bx = x0 + blx, by = y0 + bly, bz = z0 + blz;        // unwrapped
+            const int bxw = wrapc(bx, g.n[0]), byw = wrapc(by, g.n[1]), bzw = wrapc(bz, g.n[2]);
+            bool sunk = false;
+            int dex = 0, dey = 0, dez = 0;
+            if (MODE == MODE_PUSH) {
+                float Ex, Ey, Ez, Bx, By, Bz;
+                if (DIPOLE) {
+                    Ex = Ey = Ez = 0.0f;
+                    dipole_field(g, bxw, byw, bzw, px, py, pz, Bx, By, Bz);
+                } else if ((unsigned)blx < (unsigned)G.T0 && (unsigned)bly < (unsigned)G.T1 &&
+                           (unsigned)blz < (unsigned)G.T2)
+                    gather_lds(sF4, sF2, blx * fsx + bly * fsy + blz, fsx, fsy, px, py, pz, Ex, Ey, Ez, Bx, By, Bz);
+                else
+                    gather_direct(A, bxw, byw, bzw, px, py, pz, Ex, Ey, Ez, Bx, By, Bz);
+                boris_move(g, qom, subk, Ex, Ey, Ez, Bx, By, Bz, vx, vy, vz, px, py, pz);
+                if (REFLECT) reflect_faces(g, bxw, byw, bzw, px, py, pz, vx, vy, vz);
+                sunk = !finite3(px, py, pz);
+                if (!sunk) {
+                    dex = dcell(px); dey = dcell(py); dez = dcell(pz);
+                }
+                if (SINK && !sunk) {
+                    const float rx_ = g.origin[0] + ((float)wrapc(bx + dex, g.n[0]) + px - (float)dex) * g.w[0] - g.sink_c[0];
+                    const float ry_ = g.origin[1] + ((float)wrapc(by + dey, g.n[1]) + py - (float)dey) * g.w[1] - g.sink_c[1];
+                    const float rz_ = g.origin[2] + ((float)wrapc(bz + dez, g.n[2]) + pz - (float)dez) * g.w[2] - g.sink_c[2];
+                    sunk = (rx_ * rx_ + ry_ * ry_ + rz_ * rz_) <= g.sink_r2;
+                }
+            }
+            bool ok = true;
+            if (MODE != MODE_DEPOSIT) {
+                const int bc = G.cid(g, bxw, byw, bzw);
+                const int oidx = S.start[bc] + S.stay[bc] + atomicAdd(&S.arr[bc], 1);
+                if (oidx < 0 || oidx >= S.start[bc + 1] || (long long)oidx >= S.cap) {
+                    atomicOr(A.err, 1);
+                    ok = false;
+                } else {
+                    store_particle(rout, cs, oidx, sunk ? NaNf : px, py, pz, vx, vy, vz);
+                    if (TRACK) S.id_out[oidx] = S.id_in[inq];
+                }
+            }
+            if (MODE == MODE_PUSH && sunk) {
+                atomicAdd(A.removed, 1ull);
+            } else if (MODE != MODE_REBIN && ok) {
+                float wt[8];
+                tri_weights(px - (float)dex, py - (float)dey, pz - (float)dez, wt);
+                deposit_direct(A, G, sJ, x0, y0, z0, blx + dex, bly + dey, blz + dez, wt, qs * vx, qs * vy,
+                               qs * vz);
+                if (MODE == MODE_PUSH) {
+                    count_add(g, G, s_cnt, S.cnt_acc, x0, y0, z0, bx + dex, by + dey, bz + dez);
+                    if ((dex | dey | dez) == 0) count_add(g, G, s_stay, S.stay_acc, x0, y0, z0, bx, by, bz);
+                }
+            }
+        };
+        // leavers: deposit and count at the new cell, one per lane; empties the buffer
+        int kn = 0;
+        auto drain_leavers = [&]() {
+            const int kq_n = min(kn, K0CAP);
+            for (int qi = lane; qi < kq_n; qi += WS) {
+                const float* e = K0 + qi * 8;
+                const float px = e[0], py = e[1], pz = e[2], vx = e[3], vy = e[4], vz = e[5];
+                const int kc = __float_as_int(e[6]);  // the tile cell it left
+                int qlx, qly, qlz;
+                G.split(kc, qlx, qly, qlz);
+                const int dex = dcell(px), dey = dcell(py), dez = dcell(pz);
+                float wt[8];
+                tri_weights(px - (float)dex, py - (float)dey, pz - (float)dez, wt);
+                deposit_direct(A, G, sJ, x0, y0, z0, qlx + dex, qly + dey, qlz + dez, wt, qs * vx, qs * vy, qs * vz);
+                count_add(g, G, s_cnt, S.cnt_acc, x0, y0, z0, x0 + qlx + dex, y0 + qly + dey, z0 + qlz + dez);
+            }
+            wave_sync();
+            kn = 0;
+        };
+        // processes the queue's first lim entries, one per lane, and moves the rest to the front
+        int qn = 0;
+        auto drain = [&](int lim) {
+            for (int qi = lane; qi < lim; qi += WS) arrive(Q[qi]);
+            if (lim > 0) {
+                const int rest = qn - lim;
+                const int v = lane < rest ? Q[lim + lane] : 0;
+                wave_sync();
+                if (lane < rest) Q[lane] = v;
+                wave_sync();
+                qn = rest;
+            }
+        };
 
         while (c_cur < chi) {
             // runs of the next 64 cells (lane l looks at cell c_cur + l)
@@ -760,7 +862,7 @@ __global__ __launch_bounds__(NTP) IPIC_PUSH_ATTR void k_particles(PushArgs A) {
             const int sfirst = __shfl(sex, R[k], WS);
             const int base = (k == 0 ? carry : 0) + sex - sfirst;
 
-            int run_in = 0, rank = 0, qn = 0, kn = 0;
+            int run_in = 0, rank = 0;
             const int a_start = (MODE != MODE_DEPOSIT && active) ? S.start[cbase + c] : 0;
             // slots of the run cell's stayers lie in [0, a_lim): below its end and the capacity
             const int a_lim = (MODE != MODE_DEPOSIT && active) ? (int)min((long long)S.start[cbase + c + 1], S.cap) : 0;
@@ -828,7 +930,7 @@ __global__ __launch_bounds__(NTP) IPIC_PUSH_ATTR void k_particles(PushArgs A) {
                 const float px = rx[jj], py = ry[jj], pz = rz[jj];
                 const float vx = rvx[jj], vy = rvy[jj], vz = rvz[jj];
                 const bool home = (hm >> jj) & 1u;
-                if (!home && alive(px, py, pz)) qe = (pst + jj - plo) | (k << 9);  // sits in another cell
+                if (!home && alive(px, py, pz)) qe = pst + jj;  // sits in another cell
                 if (home) {
                     // a home particle's x offset is NaN only when pass B1 removed it
                     const bool sunk = (MODE == MODE_PUSH) && __builtin_isnan(px);
@@ -863,7 +965,7 @@ __global__ __launch_bounds__(NTP) IPIC_PUSH_ATTR void k_particles(PushArgs A) {
                 }
                 // ballot compaction of both queues (every lane executes this)
                 const unsigned long long bq = __ballot(qe >= 0);
-                if (qe >= 0) Q[qn + ballot_rank(bq)] = (unsigned short)qe;
+                if (qe >= 0) Q[qn + ballot_rank(bq)] = qe;
                 qn += __popcll(bq);
                 const unsigned long long bk = __ballot(leaver);
                 if (leaver) {
@@ -871,7 +973,7 @@ __global__ __launch_bounds__(NTP) IPIC_PUSH_ATTR void k_particles(PushArgs A) {
                     if (slot < K0CAP) {
                         float* e = K0 + slot * 8;
                         e[0] = px; e[1] = py; e[2] = pz; e[3] = vx; e[4] = vy; e[5] = vz;
-                        e[6] = __int_as_float(k);
+                        e[6] = __int_as_float(c);
                     } else {
                         // buffer full: handle in place
                         const int dex = dcell(px), dey = dcell(py), dez = dcell(pz);
@@ -906,85 +1008,10 @@ __global__ __launch_bounds__(NTP) IPIC_PUSH_ATTR void k_particles(PushArgs A) {
             }
             wave_sync();
 
-            // leavers: deposit and count at the new cell, one per lane
-            const int kq_n = min(kn, K0CAP);
-            for (int qi = lane; qi < kq_n; qi += WS) {
-                const float* e = K0 + qi * 8;
-                const float px = e[0], py = e[1], pz = e[2], vx = e[3], vy = e[4], vz = e[5];
-                const int kc = c_cur + __float_as_int(e[6]);
-                int qlx, qly, qlz;
-                G.split(kc, qlx, qly, qlz);
-                const int dex = dcell(px), dey = dcell(py), dez = dcell(pz);
-                float wt[8];
-                tri_weights(px - (float)dex, py - (float)dey, pz - (float)dez, wt);
-                deposit_direct(A, G, sJ, x0, y0, z0, qlx + dex, qly + dey, qlz + dez, wt, qs * vx, qs * vy, qs * vz);
-                count_add(g, G, s_cnt, S.cnt_acc, x0, y0, z0, x0 + qlx + dex, y0 + qly + dey, z0 + qlz + dez);
-            }
-            // particles sitting in another cell b: full processing, one per lane
-            for (int qi = lane; qi < qn; qi += WS) {
-                const int ent = Q[qi];
-                const int inq = plo + (ent & 511);
-                const int kc = c_cur + (ent >> 9);
-                int qlx, qly, qlz;
-                G.split(kc, qlx, qly, qlz);
-                const unsigned vq = (unsigned)inq * 4u;
-                const float ox = ldp(rin, vq, 0), oy = ldp(rin, vq, cs), oz = ldp(rin, vq, 2 * cs);
-                float vx = ldp(rin, vq, 3 * cs), vy = ldp(rin, vq, 4 * cs), vz = ldp(rin, vq, 5 * cs);
-                const int dbx = dcell(ox), dby = dcell(oy), dbz = dcell(oz);
-                float px = ox - (float)dbx, py = oy - (float)dby, pz = oz - (float)dbz;
-                const int blx = qlx + dbx, bly = qly + dby, blz = qlz + dbz;  // tile-local cell b
-                const int bx = x0 + blx, by = y0 + bly, bz = z0 + blz;        // unwrapped
-                const int bxw = wrapc(bx, g.n[0]), byw = wrapc(by, g.n[1]), bzw = wrapc(bz, g.n[2]);
-                bool sunk = false;
-                int dex = 0, dey = 0, dez = 0;
-                if (MODE == MODE_PUSH) {
-                    float Ex, Ey, Ez, Bx, By, Bz;
-                    if (DIPOLE) {
-                        Ex = Ey = Ez = 0.0f;
-                        dipole_field(g, bxw, byw, bzw, px, py, pz, Bx, By, Bz);
-                    } else if ((unsigned)blx < (unsigned)G.T0 && (unsigned)bly < (unsigned)G.T1 &&
-                               (unsigned)blz < (unsigned)G.T2)
-                        gather_lds(sF4, sF2, blx * fsx + bly * fsy + blz, fsx, fsy, px, py, pz, Ex, Ey, Ez, Bx, By, Bz);
-                    else
-                        gather_direct(A, bxw, byw, bzw, px, py, pz, Ex, Ey, Ez, Bx, By, Bz);
-                    boris_move(g, qom, subk, Ex, Ey, Ez, Bx, By, Bz, vx, vy, vz, px, py, pz);
-                    if (REFLECT) reflect_faces(g, bxw, byw, bzw, px, py, pz, vx, vy, vz);
-                    sunk = !finite3(px, py, pz);
-                    if (!sunk) {
-                        dex = dcell(px); dey = dcell(py); dez = dcell(pz);
-                    }
-                    if (SINK && !sunk) {
-                        const float rx_ = g.origin[0] + ((float)wrapc(bx + dex, g.n[0]) + px - (float)dex) * g.w[0] - g.sink_c[0];
-                        const float ry_ = g.origin[1] + ((float)wrapc(by + dey, g.n[1]) + py - (float)dey) * g.w[1] - g.sink_c[1];
-                        const float rz_ = g.origin[2] + ((float)wrapc(bz + dez, g.n[2]) + pz - (float)dez) * g.w[2] - g.sink_c[2];
-                        sunk = (rx_ * rx_ + ry_ * ry_ + rz_ * rz_) <= g.sink_r2;
-                    }
-                }
-                bool ok = true;
-                if (MODE != MODE_DEPOSIT) {
-                    const int bc = G.cid(g, bxw, byw, bzw);
-                    const int oidx = S.start[bc] + S.stay[bc] + atomicAdd(&S.arr[bc], 1);
-                    if (oidx < 0 || oidx >= S.start[bc + 1] || (long long)oidx >= S.cap) {
-                        atomicOr(A.err, 1);
-                        ok = false;
-                    } else {
-                        store_particle(rout, cs, oidx, sunk ? NaNf : px, py, pz, vx, vy, vz);
-                        if (TRACK) S.id_out[oidx] = S.id_in[inq];
-                    }
-                }
-                if (MODE == MODE_PUSH && sunk) {
-                    atomicAdd(A.removed, 1ull);
-                } else if (MODE != MODE_REBIN && ok) {
-                    float wt[8];
-                    tri_weights(px - (float)dex, py - (float)dey, pz - (float)dez, wt);
-                    deposit_direct(A, G, sJ, x0, y0, z0, blx + dex, bly + dey, blz + dez, wt, qs * vx, qs * vy,
-                                   qs * vz);
-                    if (MODE == MODE_PUSH) {
-                        count_add(g, G, s_cnt, S.cnt_acc, x0, y0, z0, bx + dex, by + dey, bz + dez);
-                        if ((dex | dey | dez) == 0) count_add(g, G, s_stay, S.stay_acc, x0, y0, z0, bx, by, bz);
-                    }
-                }
-            }
+            // leavers wait in their buffer until it is half full, arrivals until a whole wave of
+            // them is there
+            if (kn >= K0CAP / 2) drain_leavers();
+            drain(qn & ~(WS - 1));
             // advance the wave's cursor past the runs of this chunk
             const int klast = __shfl(k, nruns - 1, WS);
             if (all_runs <= WS) {
@@ -1004,6 +1031,8 @@ __global__ __launch_bounds__(NTP) IPIC_PUSH_ATTR void k_particles(PushArgs A) {
             }
             wave_sync();
         }
+        if (kn > 0) drain_leavers();  // the leavers and arrivals still queued
+        drain(qn);
         __syncthreads();
         if (MODE == MODE_PUSH) {
             for (int i = tid; i < G.TC; i += NTP) {
