@@ -36,7 +36,7 @@ namespace {
 #define IPIC_PUSH_THREADS 256    // threads per block of the particle kernel
 #endif
 #ifndef IPIC_KRUN
-#define IPIC_KRUN 6              // max particles per run (one lane, held in registers)
+#define IPIC_KRUN 5              // max particles per run (one lane, held in registers)
 #endif
 #ifndef IPIC_JHALO
 #define IPIC_JHALO 0             // node layers of the tile J accumulator beyond the tile (0 or 1);
@@ -581,13 +581,9 @@ __host__ __device__ inline int field_nodes(const Geo& g) { return (g.T[0] + 1) *
 // option bits of the particle kernel
 enum { OPT_FIX8 = 1, OPT_SINK = 2, OPT_REFLECT = 4, OPT_DIPOLE = 8 };
 
-// Register budget (waves per SIMD) of a particle-kernel instance.  The plain push fits 168 VGPRs
-// (3 waves) by itself and runs 1-2 % faster without the bound; the sink and reflection checks
-// push it to 171 unless bounded; with particle ids it needs 171-177 and would spill under the
-// bound, so it keeps 2 waves.
-constexpr int push_waves(bool track, int opt) {
-    return track ? IPIC_PUSH_WAVES_PER_EU - 1 : IPIC_PUSH_WAVES_PER_EU;
-}
+// Register budget (waves per SIMD) of a particle-kernel instance: every instance fits 168 VGPRs
+// (3 waves per SIMD) with runs of 5.
+constexpr int push_waves(bool, int) { return IPIC_PUSH_WAVES_PER_EU; }
 
 #if IPIC_PUSH_WAVES_PER_EU > 1
 #define IPIC_PUSH_ATTR __attribute__((amdgpu_waves_per_eu(push_waves(TRACK, OPT))))
