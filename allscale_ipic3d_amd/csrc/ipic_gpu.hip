@@ -301,8 +301,8 @@ __device__ __forceinline__ void count_add(const Geo& g, const TG& G, int* s_arr,
 // ---------------------------------------------------------------------------------------------
 // particle kernel: PUSH (gather + Boris + rebin + deposit), DEPOSIT (J only), REBIN (sort only)
 //
-// One block per tile of cells.  The tile's cells are split among the block's waves by particle
-// count and every wave streams its cells on its own: it cuts them into runs of at most KRUN
+// One block per tile of cells (or per slice of a tile's cells on grids of few tiles).  The
+// block's cells are split among its waves by particle count and every wave streams its cells on its own: it cuts them into runs of at most KRUN
 // particles of one cell, one run per lane, and each lane loads its run straight into registers
 // (all its loads in flight at once).  The fast path covers particles that start and end the
 // step in the run's cell, in two passes over the run registers: the push (gather from the run
