@@ -819,6 +819,11 @@ __global__ __launch_bounds__(NTP) IPIC_PUSH_ATTR void k_particles(PushArgs A) {
                 if (lane == 0) atomicOr(A.err, 2);
                 break;
             }
+            // the run cell's output range [a_start, a_end), loaded with the run so that both loads
+            // are in flight together
+            const int c_run = c_cur + k;
+            const int a_start = (MODE != MODE_DEPOSIT && active) ? S.start[cbase + c_run] : 0;
+            const int a_end = (MODE != MODE_DEPOSIT && active) ? S.start[cbase + c_run + 1] : 0;
             // this lane's run into registers
             float rx[KRUN], ry[KRUN], rz[KRUN], rvx[KRUN], rvy[KRUN], rvz[KRUN];
             uint32_t rid[TRACK ? KRUN : 1];
@@ -859,9 +864,8 @@ __global__ __launch_bounds__(NTP) IPIC_PUSH_ATTR void k_particles(PushArgs A) {
             const int base = (k == 0 ? carry : 0) + sex - sfirst;
 
             int run_in = 0, rank = 0;
-            const int a_start = (MODE != MODE_DEPOSIT && active) ? S.start[cbase + c] : 0;
             // slots of the run cell's stayers lie in [0, a_lim): below its end and the capacity
-            const int a_lim = (MODE != MODE_DEPOSIT && active) ? (int)min((long long)S.start[cbase + c + 1], S.cap) : 0;
+            const int a_lim = min(a_end, (int)S.cap);  // S.cap < 2^31 (one buffer resource < 4 GiB)
 
             // pass B1, push of the home particles in place in the run registers (gather, Boris,
             // reflection, sink); a removed particle gets a NaN x offset.  The run cell's 8 corner
