@@ -171,15 +171,23 @@ __device__ __forceinline__ int xcd_tile(int bid, int ntiles) {
     return xcd < rem ? xcd * (per + 1) + slot : rem * (per + 1) + (xcd - rem) * per + slot;
 }
 
+// Inclusive prefix sum over the wave with DPP lane moves (no LDS round trips): shifts by 1, 2,
+// 4, 8 within each row of 16 lanes (lanes shifted in from outside the row read 0), then lane
+// 15's sum added to row 1 and row 3, then lane 31's sum added to rows 2 and 3.
+__device__ __forceinline__ int wave_incl_scan(int x, int) {
+    x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, true);  // row_shr:1
+    x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, true);  // row_shr:2
+    x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, true);  // row_shr:4
+    x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, true);  // row_shr:8
+    x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false);  // row_bcast:15 into rows 1, 3
+    x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false);  // row_bcast:31 into rows 2, 3
+    return x;
+}
+
 // Exclusive prefix sum over the block (NT threads); returns the prefix, sets total.
 __device__ __forceinline__ int block_exscan(int v, int* s_w, int& total) {
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    int x = v;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const int y = __shfl_up(x, d, 64);
-        if (lane >= d) x += y;
-    }
+    const int x = wave_incl_scan(v, lane);
     if (lane == 63) s_w[wid] = x;
     __syncthreads();
     int off = 0, tot = 0;
@@ -320,18 +328,6 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Inclusive prefix sum over the wave with DPP lane moves (no LDS round trips): shifts by 1, 2,
-// 4, 8 within each row of 16 lanes (lanes shifted in from outside the row read 0), then lane
-// 15's sum added to row 1 and row 3, then lane 31's sum added to rows 2 and 3.
-__device__ __forceinline__ int wave_incl_scan(int x, int) {
-    x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, true);  // row_shr:1
-    x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, true);  // row_shr:2
-    x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, true);  // row_shr:4
-    x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, true);  // row_shr:8
-    x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false);  // row_bcast:15 into rows 1, 3
-    x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false);  // row_bcast:31 into rows 2, 3
-    return x;
-}
 
 __device__ __forceinline__ int wave_max(int x) {
 #pragma unroll
