@@ -148,8 +148,8 @@ __device__ __forceinline__ int wrapc(int i, int n) {
     return r < 0 ? r + n : r;
 }
 
-// floor(a / b) for 0 <= a < 2^16 and b >= 1 through the rounded reciprocal inv_b of b: the
-// quotient is offset by 0.5/b, more than the rounding error a * 2^-22 of the product.
+// floor(a / b) for 0 <= a < 2^21 and b >= 1 through a reciprocal inv_b within one ulp of 1/b:
+// the quotient is offset by 0.5/b, more than the error (a + 0.5) * 2^-22 / b of the product.
 __device__ __forceinline__ int fdivi(int a, float inv_b) { return (int)(((float)a + 0.5f) * inv_b); }
 
 __device__ __forceinline__ int cid_of(const Geo& g, int x, int y, int z) {
@@ -808,7 +808,7 @@ __global__ __launch_bounds__(NTP) IPIC_PUSH_ATTR void k_particles(PushArgs A) {
                 const int nr = R[k + 1] - R[k];
                 const int p0 = (k == 0) ? pos_cur : s_seg[c_cur + k];
                 const int nc = s_seg[c_cur + k + 1] - p0;
-                const int bl = fdivi(nc, __builtin_amdgcn_rcpf((float)nr));
+                const int bl = nc < (1 << 21) ? fdivi(nc, __builtin_amdgcn_rcpf((float)nr)) : nc / nr;
                 const int rem = nc - bl * nr;
                 len = bl + (j < rem ? 1 : 0);
                 pst = p0 + j * bl + min(j, rem);
@@ -2227,6 +2227,8 @@ int ipic_gpu_create(const ipic_desc* desc, ipic_ctx** out) {
     const ipic_desc& d = *desc;
     for (int k = 0; k < 3; ++k) {
         if (d.n[k] <= 0) return fail(nullptr, IPIC_E_ARG, "grid size must be positive");
+        // cell coordinates go through fdivi (exact below 2^21)
+        if (d.n[k] > (1 << 21)) return fail(nullptr, IPIC_E_RANGE, "at most 2097152 cells per axis");
         if (!(d.cell_width[k] > 0)) return fail(nullptr, IPIC_E_ARG, "cell width must be positive");
     }
     if (!(d.dt > 0) || !(d.c > 0)) return fail(nullptr, IPIC_E_ARG, "dt and c must be positive");

@@ -59,7 +59,7 @@ typedef struct ipic_species_desc {
 
 /* UniverseProperties (universe_properties.h:52-142) plus the GPU layout parameters. */
 typedef struct ipic_desc {
-    int32_t n[3];                 /* cells per axis (UniverseProperties::size) */
+    int32_t n[3];                 /* cells per axis (UniverseProperties::size); each <= 2^21, product <= 2^30 */
     int32_t nspecies;             /* 1..IPIC_MAX_SPECIES */
     double cell_width[3];         /* UniverseProperties::cellWidth */
     double origin[3];             /* UniverseProperties::origin */

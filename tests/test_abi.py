@@ -75,6 +75,7 @@ def _create(**kw):
     ({"nspecies": 0}, -1, "nspecies"),
     ({"nspecies": 9}, -1, "nspecies"),
     ({"n": (2048, 1024, 1024)}, -4, "grid too large"),
+    ({"n": (1, 1, 2097153)}, -4, "cells per axis"),
     ({"field_substeps": -1}, -1, "field_substeps"),
     ({"tile": (3, 0, 0)}, -1, "tile size"),
     ({"tile": (16, 0, 0), "n": (32, 4, 4)}, -1, "tile size"),
