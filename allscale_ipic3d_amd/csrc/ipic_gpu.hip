@@ -1087,18 +1087,34 @@ __device__ __forceinline__ void xyz_of(const Geo& g, int i, int& x, int& y, int&
 
 // The cross-tile moment sum: J of node (x,y,z) summed in FP64 over every tile partial covering
 // it (every slice of the tile) plus the overflow accumulator (cleared here), rounded once to FP32.
+// The (tile, local node) pairs of axis d whose J partial covers coordinate x, in increasing
+// local index (the order of the jpairs table): without a halo, (x / T, x % T) and, on a tile's
+// low face, (the previous tile, T); with a halo, from the table.
+__device__ __forceinline__ int axis_pairs(const FieldArgs& F, int d, int x, int* p) {
+    const Geo& g = F.g;
+    if (g.jh == 0) {
+        const int t = fdivi(x, g.inv_T[d]), l = x - t * g.T[d];
+        p[0] = (t << 8) | l;
+        if (l != 0) return 1;
+        p[1] = ((t == 0 ? g.NTl[d] - 1 : t - 1) << 8) | g.T[d];
+        return 2;
+    }
+    const int* e = F.jpairs + (d * g.nmax + x) * (1 + JPMAX);
+    for (int q = 0; q < e[0]; ++q) p[q] = e[1 + q];
+    return e[0];
+}
+
 __device__ __forceinline__ float4 sum_partials(const FieldArgs& F, int i, int x, int y, int z) {
     const Geo& g = F.g;
     double sx = 0.0, sy = 0.0, sz = 0.0;
-    const int* px = F.jpairs + (0 * g.nmax + x) * (1 + JPMAX);
-    const int* py = F.jpairs + (1 * g.nmax + y) * (1 + JPMAX);
-    const int* pz = F.jpairs + (2 * g.nmax + z) * (1 + JPMAX);
-    for (int a = 0; a < px[0]; ++a)
-        for (int b = 0; b < py[0]; ++b)
-            for (int cc = 0; cc < pz[0]; ++cc) {
-                const int ta = px[1 + a] >> 8, la = px[1 + a] & 255;
-                const int tb = py[1 + b] >> 8, lb = py[1 + b] & 255;
-                const int tc = pz[1 + cc] >> 8, lc = pz[1 + cc] & 255;
+    int px[JPMAX], py[JPMAX], pz[JPMAX];
+    const int nx = axis_pairs(F, 0, x, px), ny = axis_pairs(F, 1, y, py), nz = axis_pairs(F, 2, z, pz);
+    for (int a = 0; a < nx; ++a)
+        for (int b = 0; b < ny; ++b)
+            for (int cc = 0; cc < nz; ++cc) {
+                const int ta = px[a] >> 8, la = px[a] & 255;
+                const int tb = py[b] >> 8, lb = py[b] & 255;
+                const int tc = pz[cc] >> 8, lc = pz[cc] & 255;
                 const int tile = (ta * g.NTl[1] + tb) * g.NTl[2] + tc;
                 const int l = (la * g.nj[1] + lb) * g.nj[2] + lc;
                 const float* base = F.jpart + (long long)tile * g.nsl * 3 * g.NJ;
