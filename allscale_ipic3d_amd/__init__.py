@@ -117,6 +117,33 @@ class RefDistribution(C.Structure):
     ]
 
 
+class FieldInit(C.Structure):
+    """ipic_field_init: what initFields / getDipoleFieldAt (field.h:43-118) read."""
+    _fields_ = [
+        ("drift", C.c_double * 3), ("B0", C.c_double * 3), ("object_center", C.c_double * 3),
+        ("planet_radius", C.c_double), ("external_Bz", C.c_double),
+    ]
+
+
+def field_init(drift=(0.0, 0.0, 0.0), B0=(0.0, 0.0, 0.0), object_center=(0.0, 0.0, 0.0), planet_radius=0.0,
+               external_Bz=0.0):
+    f = FieldInit()
+    f.drift[:] = [float(v) for v in drift]
+    f.B0[:] = [float(v) for v in B0]
+    f.object_center[:] = [float(v) for v in object_center]
+    f.planet_radius = float(planet_radius)
+    f.external_Bz = float(external_Bz)
+    return f
+
+
+def dipole_field_init(moment, center, radius, B0=(0.0, 0.0, 0.0), drift=(0.0, 0.0, 0.0)):
+    """The field of a z dipole of the given moment (B = moment (3xz, 3yz, 2z^2-x^2-y^2) / r^5
+    outside `radius`, zero inside) in getDipoleFieldAt's parameters: -external_Bz radius^3 =
+    moment."""
+    return field_init(drift=drift, B0=B0, object_center=center, planet_radius=radius,
+                      external_Bz=-float(moment) / float(radius) ** 3)
+
+
 def ref_distribution(pos_min, pos_max, vel_mean, vel_stddev, num_particles, seed=0, nthreads=0):
     d = RefDistribution()
     d.pos_min[:] = [float(v) for v in pos_min]
@@ -136,7 +163,8 @@ ABI_SYMBOLS = (
     "ipic_gpu_cell_counts", "ipic_gpu_download_particles", "ipic_gpu_download_fields",
     "ipic_gpu_conservation", "ipic_gpu_device_info", "ipic_gpu_field_step", "ipic_gpu_particle_step",
     "ipic_gpu_set_field_operator", "ipic_gpu_field_state", "ipic_ref_cell_counts", "ipic_ref_particles",
-    "ipic_ref_last_error", "ipic_gpu_init_reference",
+    "ipic_ref_last_error", "ipic_gpu_init_reference", "ipic_field_init_host", "ipic_field_init_last_error",
+    "ipic_gpu_init_fields",
 )
 
 _lib = None
@@ -178,8 +206,12 @@ def load_library(path: str = LIB_PATH):
     L.ipic_ref_last_error.restype = C.c_char_p
     L.ipic_ref_last_error.argtypes = []
     L.ipic_gpu_init_reference.argtypes = [P, I32, C.POINTER(RefDistribution), P]
+    L.ipic_field_init_host.argtypes = [C.POINTER(Desc), C.POINTER(FieldInit), P, P, P, P]
+    L.ipic_field_init_last_error.restype = C.c_char_p
+    L.ipic_field_init_last_error.argtypes = []
+    L.ipic_gpu_init_fields.argtypes = [P, C.POINTER(FieldInit), P]
     for name in ABI_SYMBOLS:
-        if name not in ("ipic_gpu_last_error", "ipic_ref_last_error"):
+        if name not in ("ipic_gpu_last_error", "ipic_ref_last_error", "ipic_field_init_last_error"):
             getattr(L, name).restype = C.c_int
     _lib = L
     return L
@@ -213,6 +245,32 @@ def reference_init(desc, dist, particles=True):
     if L.ipic_ref_particles(C.byref(desc), C.byref(dist), _ptr(counts), _ptr(pos), _ptr(vel)) != 0:
         raise IpicError(L.ipic_ref_last_error().decode())
     return counts, pos, vel
+
+
+def field_init_host(desc, init):
+    """initFields + initBcFields on the host (ipic_field_init_host): E and node B at the N^3
+    nodes, Bc at the N^3 centres (row-major, n x 3 doubles) and the logged initial field
+    energies (getFieldEnergy of E and of B + Bext over the (N+1)^3 nodes).  No GPU is used."""
+    L = load_library()
+    ncell = int(desc.n[0]) * int(desc.n[1]) * int(desc.n[2])
+    E, Bn, Bc = (np.zeros((ncell, 3)) for _ in range(3))
+    en = np.zeros(2)
+    if L.ipic_field_init_host(C.byref(desc), C.byref(init), _ptr(E), _ptr(Bn), _ptr(Bc), _ptr(en)) != 0:
+        raise IpicError(L.ipic_field_init_last_error().decode())
+    return {"E": E, "Bn": Bn, "Bc": Bc, "energies": (float(en[0]), float(en[1]))}
+
+
+def desc_for(n, cell_width=(1.0, 1.0, 1.0), origin=(0.0, 0.0, 0.0)):
+    """An ipic_desc of the grid geometry only (for the host-side set-up functions)."""
+    if np.isscalar(n):
+        n = (n, n, n)
+    d = Desc()
+    d.n[:] = [int(v) for v in n]
+    d.nspecies = 1
+    d.cell_width[:] = [float(v) for v in cell_width]
+    d.origin[:] = [float(v) for v in origin]
+    d.dt, d.c = 1.0, 1.0
+    return d
 
 
 def device_info():
@@ -310,6 +368,13 @@ class Simulation:
         counts = np.zeros(self.ncell, dtype=np.int64)
         self._check(self._L.ipic_gpu_init_reference(self._h, int(species), C.byref(dist), _ptr(counts)))
         return counts
+
+    def init_fields(self, init):
+        """initFields + initBcFields (field.h:81-138) as the field state (ipic_gpu_init_fields);
+        returns the logged initial field energies (E, B + Bext)."""
+        en = np.zeros(2)
+        self._check(self._L.ipic_gpu_init_fields(self._h, C.byref(init), _ptr(en)))
+        return float(en[0]), float(en[1])
 
     def upload_fields(self, E=None, Bc=None):
         e = None if E is None else _f64(E, (self.ncell, 3))

@@ -21,7 +21,8 @@ HIPFLAGS = [
 ]
 
 LIBS = {
-    "libipic_gpu.so": ["csrc/ipic_gpu.hip", "csrc/ipic_nop.hip", "csrc/ipic_refinit.cpp"],
+    "libipic_gpu.so": ["csrc/ipic_gpu.hip", "csrc/ipic_nop.hip", "csrc/ipic_refinit.cpp",
+                       "csrc/ipic_fieldinit.cpp"],
 }
 
 

@@ -24,6 +24,7 @@
 #include <cstring>
 #include <string>
 #include <exception>
+#include <new>
 #include <vector>
 
 #include "ipic_gpu.h"
@@ -121,7 +122,7 @@ struct PushArgs {
     const float4* E;
     const float4* Bn;
     float* jpart;
-    float* jover;
+    long long* jover;     // overflow deposits, 64-bit fixed point like the tile accumulator
     int* jflag;
     unsigned long long* removed;
     int* err;             // set to nonzero when a consistency guard fires
@@ -208,13 +209,20 @@ __device__ __forceinline__ int block_exscan(int v, int* s_w, int& total) {
 // integer x in one FP64 fused multiply-add with 1.5 * 2^52, whose unit in the last place is 1:
 // for |x| < 2^51 the bit pattern of the result is bits(1.5 * 2^52) + x, and as the constant's
 // low word is zero, x is that pattern with the constant's high word subtracted.  v is clamped
-// to |x| <= 2^50 first (NaN to the lower bound), so the identity always holds.
-__device__ __forceinline__ void fix_add(long long* p, float v, float lim, double scale) {
+// to |x| <= 2^50 first (NaN to the lower bound), so the identity always holds; the callers
+// raise ERR_JSAT when a value reaches the clamp.
+__device__ __forceinline__ long long fix_of(float v, float lim, double scale) {
     const float vc = fminf(fmaxf(v, -lim), lim);
     const double d = fma((double)vc, scale, 6755399441055744.0);
-    const long long q = (long long)((unsigned long long)__double_as_longlong(d) - 0x4338000000000000ull);
-    __hip_atomic_fetch_add(p, q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    return (long long)((unsigned long long)__double_as_longlong(d) - 0x4338000000000000ull);
 }
+
+__device__ __forceinline__ void fix_add(long long* p, float v, float lim, double scale) {
+    __hip_atomic_fetch_add(p, fix_of(v, lim, scale), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// device error bits (the host turns them into an error of the next synchronising call)
+enum { ERR_SLOT = 1, ERR_CHUNK = 2, ERR_JSAT = 4 };
 
 // Trilinear weights (cell.h:876-880): w[(i*2+j)*2+k] = (i?x:1-x)(j?y:1-y)(k?z:1-z)
 __device__ __forceinline__ void tri_weights(float x, float y, float z, float wt[8]) {
@@ -285,14 +293,22 @@ __device__ __forceinline__ void deposit_direct(const PushArgs& A, const TG& G, l
                     fix_add(&sJ[G.NJ + l], jy * ww, g.fix_lim, g.fix_scale);
                     fix_add(&sJ[2 * G.NJ + l], jz * ww, g.fix_lim, g.fix_scale);
                 } else {
+                    // beyond the tile: the global accumulator, in the same fixed point, so J does
+                    // not depend on the order of these adds either
                     const int gid = nid_of(g, wrapc(x0 + elx + i, g.n[0]), wrapc(y0 + ely + j, g.n[1]),
                                            wrapc(z0 + elz + k, g.n[2]));
-                    atomicAdd(&A.jover[3 * gid + 0], jx * ww);
-                    atomicAdd(&A.jover[3 * gid + 1], jy * ww);
-                    atomicAdd(&A.jover[3 * gid + 2], jz * ww);
+                    long long* o = A.jover + 3 * (long long)gid;
+                    __hip_atomic_fetch_add(o + 0, fix_of(jx * ww, g.fix_lim, g.fix_scale), __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+                    __hip_atomic_fetch_add(o + 1, fix_of(jy * ww, g.fix_lim, g.fix_scale), __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+                    __hip_atomic_fetch_add(o + 2, fix_of(jz * ww, g.fix_lim, g.fix_scale), __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
                     *A.jflag = 1;
                 }
             }
+    // every corner value is at most max |j| (the weights lie in [0, 1])
+    if (fmaxf(fabsf(jx), fmaxf(fabsf(jy), fabsf(jz))) > g.fix_lim) atomicOr(A.err, ERR_JSAT);
 }
 
 template <class TG>
@@ -724,7 +740,7 @@ __global__ __launch_bounds__(NTP) IPIC_PUSH_ATTR void k_particles(PushArgs A) {
                 const int bc = G.cid(g, bxw, byw, bzw);
                 const int oidx = S.start[bc] + S.stay[bc] + atomicAdd(&S.arr[bc], 1);
                 if (oidx < 0 || oidx >= S.start[bc + 1] || (long long)oidx >= S.cap) {
-                    atomicOr(A.err, 1);
+                    atomicOr(A.err, ERR_SLOT);
                     ok = false;
                 } else {
                     store_particle(rout, cs, oidx, sunk ? NaNf : px, py, pz, vx, vy, vz);
@@ -816,7 +832,7 @@ __global__ __launch_bounds__(NTP) IPIC_PUSH_ATTR void k_particles(PushArgs A) {
             const int plo = __shfl(pst, 0, WS);
             const int phi = __shfl(pst + len, nruns - 1, WS);
             if (plo < 0 || (long long)phi > S.cap || phi - plo > WCH || phi < plo) {  // uniform over the wave
-                if (lane == 0) atomicOr(A.err, 2);
+                if (lane == 0) atomicOr(A.err, ERR_CHUNK);
                 break;
             }
             // the run cell's output range [a_start, a_end), loaded with the run so that both loads
@@ -923,6 +939,7 @@ __global__ __launch_bounds__(NTP) IPIC_PUSH_ATTR void k_particles(PushArgs A) {
             float mom[3][8];
 #pragma unroll
             for (int m = 0; m < 8; ++m) mom[0][m] = mom[1][m] = mom[2][m] = 0.0f;
+            float jmax = 0.0f;  // largest |j| component of the run: bounds every corner sum / run_in
 #pragma unroll
             for (int jj = 0; jj < KRUN; ++jj) {
                 int qe = -1;       // arrival queue entry of this lane
@@ -938,7 +955,7 @@ __global__ __launch_bounds__(NTP) IPIC_PUSH_ATTR void k_particles(PushArgs A) {
                         const int oidx = a_start + base + rank;
                         ++rank;
                         if ((unsigned)oidx >= (unsigned)a_lim) {
-                            atomicOr(A.err, 1);
+                            atomicOr(A.err, ERR_SLOT);
                         } else {
                             store_particle(rout, cs, oidx, px, py, pz, vx, vy, vz);
                             if (TRACK) S.id_out[oidx] = rid[jj];
@@ -949,6 +966,7 @@ __global__ __launch_bounds__(NTP) IPIC_PUSH_ATTR void k_particles(PushArgs A) {
                     } else if (MODE != MODE_REBIN) {
                         if (MODE == MODE_DEPOSIT || in_cell(px, py, pz)) {
                             const float jx = qs * vx, jy = qs * vy, jz = qs * vz;
+                            jmax = fmaxf(jmax, fmaxf(fabsf(jx), fmaxf(fabsf(jy), fabsf(jz))));
                             const float xy = px * py, xz = px * pz, yz = py * pz, xyz = xy * pz;
                             const float mono[8] = {1.0f, px, py, pz, xy, xz, yz, xyz};
 #pragma unroll
@@ -987,6 +1005,7 @@ __global__ __launch_bounds__(NTP) IPIC_PUSH_ATTR void k_particles(PushArgs A) {
                 kn += __popcll(bk);
             }
             if (MODE != MODE_REBIN && run_in > 0) {
+                if (jmax * (float)run_in > g.fix_lim) atomicOr(A.err, ERR_JSAT);
 #pragma unroll
                 for (int d = 0; d < 3; ++d) {
                     float N[8];
@@ -1070,7 +1089,7 @@ struct FieldArgs {
     const float4* SE;     // static E source (nullable)
     const float4* SB;     // static B source (nullable)
     const float* jpart;
-    float* jover;
+    long long* jover;
     int* jflag;
     const int* jpairs;  // [3][nmax][1+JPMAX]: count, then (tile << 8 | local)
     float dt;           // field substep
@@ -1125,12 +1144,13 @@ __device__ __forceinline__ float4 sum_partials(const FieldArgs& F, int i, int x,
                 }
             }
     if (*F.jflag) {
-        sx += (double)F.jover[3 * i + 0];
-        sy += (double)F.jover[3 * i + 1];
-        sz += (double)F.jover[3 * i + 2];
-        F.jover[3 * i + 0] = 0.0f;
-        F.jover[3 * i + 1] = 0.0f;
-        F.jover[3 * i + 2] = 0.0f;
+        long long* o = F.jover + 3 * (long long)i;
+        sx += (double)o[0] * g.fix_inv;
+        sy += (double)o[1] * g.fix_inv;
+        sz += (double)o[2] * g.fix_inv;
+        o[0] = 0;
+        o[1] = 0;
+        o[2] = 0;
     }
     return make_float4((float)sx, (float)sy, (float)sz, 0.0f);
 }
@@ -1148,10 +1168,10 @@ __global__ __launch_bounds__(256) void k_jsum(FieldArgs F) {
 __global__ void k_clear_flag(int* f) { *f = 0; }
 
 // Drops overflow deposits that were never summed (a deposit pass replaces the partials).
-__global__ void k_discard_overflow(const int* flag, float* jover, long long n) {
+__global__ void k_discard_overflow(const int* flag, long long* jover, long long n) {
     if (!*flag) return;
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) jover[i] = 0.0f;
+    if (i < n) jover[i] = 0;
 }
 
 // E pass of field.h:359-363: E += dt (c^2 curl- Bc - J).  With from_partials J is first
@@ -1664,7 +1684,7 @@ struct ipic_ctx {
     float4* SB = nullptr;
     std::vector<double> bgE, bgBc;  // background (state at the last field upload), FP64
     float* jpart = nullptr;
-    float* jover = nullptr;
+    long long* jover = nullptr;
     int* jflag = nullptr;
     int* jpairs = nullptr;
     unsigned long long* removed = nullptr;
@@ -1897,7 +1917,12 @@ int launch_scan(ipic_ctx* ctx) {
     return 0;
 }
 
+int launch_jsum(ipic_ctx* ctx);
+
 int launch_leapfrog(ipic_ctx* ctx) {
+    // the static field (no substeps) still forms J from the deposit, so J and the overflow
+    // accumulator describe the current state as after a leapfrog phase
+    if (ctx->d.field_substeps == 0) return launch_jsum(ctx);
     FieldArgs F = field_args(ctx);
     const int nb = (ctx->g.ncell + 255) / 256;
     for (int s = 0; s < ctx->d.field_substeps; ++s) {
@@ -2198,6 +2223,10 @@ int check_device_flag(ipic_ctx* ctx) {
     if (v) {
         const int z = 0;
         HIPCK(ctx, hipMemcpy(ctx->derr, &z, sizeof(int), hipMemcpyHostToDevice));
+        if (v == ERR_JSAT)
+            return fail(ctx, IPIC_E_STATE,
+                        "current density out of the fixed-point range (a particle's q v exceeds 2^20 q c): "
+                        "J is clamped and the state is invalid");
         return fail(ctx, IPIC_E_STATE,
                     std::string("particle layout guard fired (code ") + std::to_string(v) +
                         "): per-cell counts inconsistent; state is invalid");
@@ -2208,6 +2237,21 @@ int check_device_flag(ipic_ctx* ctx) {
 int check_ctx(ipic_ctx* ctx) {
     if (!ctx) return fail(nullptr, IPIC_E_ARG, "null context");
     return 0;
+}
+
+// Runs the body of a C-ABI entry point: no C++ exception (a failed host allocation, for
+// instance) crosses the ABI; it becomes an error code with a message instead.
+template <class F>
+int guarded(ipic_ctx* ctx, F&& body) {
+    try {
+        return body();
+    } catch (const std::bad_alloc&) {
+        return fail(ctx, IPIC_E_RANGE, "out of host memory");
+    } catch (const std::exception& e) {
+        return fail(ctx, IPIC_E_HIP, std::string("internal error: ") + e.what());
+    } catch (...) {
+        return fail(ctx, IPIC_E_HIP, "internal error");
+    }
 }
 
 }  // namespace
@@ -2238,712 +2282,759 @@ int ipic_gpu_device_info(char* name, int32_t name_len, int32_t* cu_count, int64_
 }
 
 int ipic_gpu_create(const ipic_desc* desc, ipic_ctx** out) {
-    if (!desc || !out) return fail(nullptr, IPIC_E_ARG, "null argument");
-    *out = nullptr;
-    const ipic_desc& d = *desc;
-    for (int k = 0; k < 3; ++k) {
-        if (d.n[k] <= 0) return fail(nullptr, IPIC_E_ARG, "grid size must be positive");
-        // cell coordinates go through fdivi (exact below 2^21)
-        if (d.n[k] > (1 << 21)) return fail(nullptr, IPIC_E_RANGE, "at most 2097152 cells per axis");
-        if (!(d.cell_width[k] > 0)) return fail(nullptr, IPIC_E_ARG, "cell width must be positive");
-    }
-    if (!(d.dt > 0) || !(d.c > 0)) return fail(nullptr, IPIC_E_ARG, "dt and c must be positive");
-    if (d.nspecies < 1 || d.nspecies > IPIC_MAX_SPECIES) return fail(nullptr, IPIC_E_ARG, "nspecies out of range");
-    const long long ncell = (long long)d.n[0] * d.n[1] * d.n[2];
-    if (ncell > (1LL << 30)) return fail(nullptr, IPIC_E_RANGE, "grid too large");
-
-    ipic_ctx* ctx = new ipic_ctx();
-    ctx->d = d;
-    if (ctx->d.field_substeps < 0) {
-        delete ctx;
-        return fail(nullptr, IPIC_E_ARG, "field_substeps must be >= 0 (0: static field)");
-    }
-    ctx->track = (d.flags & IPIC_FLAG_TRACK_IDS) != 0;
-    Geo& g = ctx->g;
-    for (int k = 0; k < 3; ++k) {
-        g.n[k] = d.n[k];
-        int T = d.tile[k] > 0 ? d.tile[k] : largest_divisor_le(d.n[k], 8);
-        if (T > 8 || d.n[k] % T != 0) {
-            delete ctx;
-            return fail(nullptr, IPIC_E_ARG, "tile size must divide the grid and be at most 8");
-        }
-        g.T[k] = T;
-        g.NTl[k] = d.n[k] / T;
-        g.jh = IPIC_JHALO;
-        g.nj[k] = T + 1 + 2 * g.jh;
-        g.w[k] = (float)d.cell_width[k];
-        g.inv_w[k] = (float)(1.0 / d.cell_width[k]);
-        g.origin[k] = (float)d.origin[k];
-        g.sink_c[k] = (float)d.sink_center[k];
-    }
-    g.TC = g.T[0] * g.T[1] * g.T[2];
-    g.ntiles = g.NTl[0] * g.NTl[1] * g.NTl[2];
-    {
-        // One block per tile leaves most CUs idle on a grid of few tiles (8^3 cells: one tile).
-        // The particle kernel then splits each tile's cells among several blocks, up to two
-        // blocks per CU and at least as many cells per block as it has waves.
-        int cus = 256, dev = 0;
-        if (hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
-            cus = 256;
-        const int want = (2 * cus + g.ntiles - 1) / g.ntiles;
-        g.nsl = std::max(1, std::min(want, g.TC / NW));
-    }
-    g.ncell = (int)ncell;
-    g.NJ = g.nj[0] * g.nj[1] * g.nj[2];
-    g.nmax = std::max(g.n[0], std::max(g.n[1], g.n[2]));
-    g.s0 = g.n[1] * g.n[2];
-    g.dt = (float)d.dt;
-    g.inv_vol = (float)(1.0 / (d.cell_width[0] * d.cell_width[1] * d.cell_width[2]));
-    g.pi_c = (float)(M_PI * d.c);
-    for (int k = 0; k < 3; ++k) g.inv_T[k] = 1.0f / (float)g.T[k];
-    g.inv_T12 = 1.0f / (float)(g.T[1] * g.T[2]);
-    {
-        // fixed-point scale of the LDS J accumulator: one particle at speed c contributes about
-        // 2^30 units, leaving 2^33 of headroom for the sums of a tile node
-        const double vol = d.cell_width[0] * d.cell_width[1] * d.cell_width[2];
-        double qmax = 0.0;
-        for (int s = 0; s < d.nspecies; ++s) qmax = std::max(qmax, std::fabs(d.species[s].q) / (8.0 * vol));
-        int bits = 40;
-        if (qmax * d.c > 0.0 && std::isfinite(qmax * d.c)) bits = 30 - (int)std::ceil(std::log2(qmax * d.c));
-        bits = std::max(-60, std::min(120, bits));
-        g.fix_lim = (float)std::ldexp(1.0, 50 - bits);
-        g.fix_scale = std::ldexp(1.0, bits);
-        g.fix_inv = std::ldexp(1.0, -bits);
-    }
-    g.sink = d.sink ? 1 : 0;
-    g.reflect = (d.flags & IPIC_FLAG_REFLECT) ? 1 : 0;
-    {
-        const double L = d.cell_width[0];
-        g.dip_m = (float)(d.dipole_moment / (L * L * L));
+    return guarded(nullptr, [&]() -> int {
+        if (!desc || !out) return fail(nullptr, IPIC_E_ARG, "null argument");
+        *out = nullptr;
+        const ipic_desc& d = *desc;
         for (int k = 0; k < 3; ++k) {
-            g.dip_o[k] = (float)(d.origin[k] / L);
-            g.dip_w[k] = (float)(d.cell_width[k] / L);
+            if (d.n[k] <= 0) return fail(nullptr, IPIC_E_ARG, "grid size must be positive");
+            // cell coordinates go through fdivi (exact below 2^21)
+            if (d.n[k] > (1 << 21)) return fail(nullptr, IPIC_E_RANGE, "at most 2097152 cells per axis");
+            if (!(d.cell_width[k] > 0)) return fail(nullptr, IPIC_E_ARG, "cell width must be positive");
         }
-    }
-    g.sink_r2 = (float)(d.sink_radius * d.sink_radius);
+        if (!(d.dt > 0) || !(d.c > 0)) return fail(nullptr, IPIC_E_ARG, "dt and c must be positive");
+        if (d.nspecies < 1 || d.nspecies > IPIC_MAX_SPECIES) return fail(nullptr, IPIC_E_ARG, "nspecies out of range");
+        const long long ncell = (long long)d.n[0] * d.n[1] * d.n[2];
+        if (ncell > (1LL << 30)) return fail(nullptr, IPIC_E_RANGE, "grid too large");
 
-    std::vector<int> jp;
-    if (build_jpairs(g, jp) != 0) {
-        delete ctx;
-        return fail(nullptr, IPIC_E_ARG, "tile/grid combination not supported");
-    }
-    hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
-    if (e != hipSuccess) {
-        delete ctx;
-        return fail(nullptr, IPIC_E_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
-    }
-    int rc = 0;
-    const size_t nc = (size_t)g.ncell;
-    rc = rc ? rc : dalloc(ctx, &ctx->E, nc);
-    rc = rc ? rc : dalloc(ctx, &ctx->Bc, nc);
-    rc = rc ? rc : dalloc(ctx, &ctx->Bn, nc);
-    rc = rc ? rc : dalloc(ctx, &ctx->J, nc);
-    rc = rc ? rc : dalloc(ctx, &ctx->jpart, (size_t)g.ntiles * g.nsl * 3 * g.NJ);
-    rc = rc ? rc : dalloc(ctx, &ctx->jover, nc * 3);
-    rc = rc ? rc : dalloc(ctx, &ctx->jflag, 1);
-    rc = rc ? rc : dalloc(ctx, &ctx->jpairs, jp.size());
-    rc = rc ? rc : dalloc(ctx, &ctx->removed, 1);
-    rc = rc ? rc : dalloc(ctx, &ctx->derr, 1);
-    rc = rc ? rc : dalloc(ctx, &ctx->tile_order, (size_t)g.ntiles);
-    rc = rc ? rc : dalloc(ctx, &ctx->red, (size_t)ctx->red_blocks * 5);
-    for (int s = 0; s < d.nspecies && !rc; ++s) {
-        Species& S = ctx->sp[s];
-        S.q = d.species[s].q;
-        S.qom = d.species[s].qom;
-        rc = rc ? rc : dalloc(ctx, &S.seg, nc + 1);
-        rc = rc ? rc : dalloc(ctx, &S.start, nc + 1);
-        rc = rc ? rc : dalloc(ctx, &S.stay, nc);
-        rc = rc ? rc : dalloc(ctx, &S.cnt_acc, nc);
-        rc = rc ? rc : dalloc(ctx, &S.stay_acc, nc);
-        rc = rc ? rc : dalloc(ctx, &S.arr, nc);
-        rc = rc ? rc : dalloc(ctx, &S.tile_total, (size_t)g.ntiles);
-        rc = rc ? rc : dalloc(ctx, &S.tile_base, (size_t)g.ntiles + 1);
-    }
-    if (!rc) {
-        e = hipMemcpyAsync(ctx->jpairs, jp.data(), jp.size() * sizeof(int), hipMemcpyHostToDevice, ctx->stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-        if (e != hipSuccess) rc = fail(ctx, IPIC_E_HIP, std::string("upload: ") + hipGetErrorString(e));
-    }
-    if (!rc)
-        for (int k = 0; k < 8; ++k)
-            if (hipEventCreate(&ctx->ev[k]) != hipSuccess) rc = fail(ctx, IPIC_E_HIP, "hipEventCreate");
-    if (rc) {
-        std::string msg = ctx->err;
-        ipic_gpu_destroy(ctx);
-        g_last_error = msg;
-        return rc;
-    }
-    *out = ctx;
-    return 0;
+        ipic_ctx* ctx = new ipic_ctx();
+        ctx->d = d;
+        if (ctx->d.field_substeps < 0) {
+            delete ctx;
+            return fail(nullptr, IPIC_E_ARG, "field_substeps must be >= 0 (0: static field)");
+        }
+        ctx->track = (d.flags & IPIC_FLAG_TRACK_IDS) != 0;
+        Geo& g = ctx->g;
+        for (int k = 0; k < 3; ++k) {
+            g.n[k] = d.n[k];
+            int T = d.tile[k] > 0 ? d.tile[k] : largest_divisor_le(d.n[k], 8);
+            if (T > 8 || d.n[k] % T != 0) {
+                delete ctx;
+                return fail(nullptr, IPIC_E_ARG, "tile size must divide the grid and be at most 8");
+            }
+            g.T[k] = T;
+            g.NTl[k] = d.n[k] / T;
+            g.jh = IPIC_JHALO;
+            g.nj[k] = T + 1 + 2 * g.jh;
+            g.w[k] = (float)d.cell_width[k];
+            g.inv_w[k] = (float)(1.0 / d.cell_width[k]);
+            g.origin[k] = (float)d.origin[k];
+            g.sink_c[k] = (float)d.sink_center[k];
+        }
+        g.TC = g.T[0] * g.T[1] * g.T[2];
+        g.ntiles = g.NTl[0] * g.NTl[1] * g.NTl[2];
+        {
+            // One block per tile leaves most CUs idle on a grid of few tiles (8^3 cells: one tile).
+            // The particle kernel then splits each tile's cells among several blocks, up to two
+            // blocks per CU and at least as many cells per block as it has waves.
+            int cus = 256, dev = 0;
+            if (hipGetDevice(&dev) != hipSuccess ||
+                hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+                cus = 256;
+            const int want = (2 * cus + g.ntiles - 1) / g.ntiles;
+            g.nsl = std::max(1, std::min(want, g.TC / NW));
+        }
+        g.ncell = (int)ncell;
+        g.NJ = g.nj[0] * g.nj[1] * g.nj[2];
+        g.nmax = std::max(g.n[0], std::max(g.n[1], g.n[2]));
+        g.s0 = g.n[1] * g.n[2];
+        g.dt = (float)d.dt;
+        g.inv_vol = (float)(1.0 / (d.cell_width[0] * d.cell_width[1] * d.cell_width[2]));
+        g.pi_c = (float)(M_PI * d.c);
+        for (int k = 0; k < 3; ++k) g.inv_T[k] = 1.0f / (float)g.T[k];
+        g.inv_T12 = 1.0f / (float)(g.T[1] * g.T[2]);
+        {
+            // fixed-point scale of the LDS J accumulator: one particle at speed c contributes about
+            // 2^30 units, leaving 2^33 of headroom for the sums of a tile node
+            const double vol = d.cell_width[0] * d.cell_width[1] * d.cell_width[2];
+            double qmax = 0.0;
+            for (int s = 0; s < d.nspecies; ++s) qmax = std::max(qmax, std::fabs(d.species[s].q) / (8.0 * vol));
+            int bits = 40;
+            if (qmax * d.c > 0.0 && std::isfinite(qmax * d.c)) bits = 30 - (int)std::ceil(std::log2(qmax * d.c));
+            bits = std::max(-60, std::min(120, bits));
+            g.fix_lim = (float)std::ldexp(1.0, 50 - bits);
+            g.fix_scale = std::ldexp(1.0, bits);
+            g.fix_inv = std::ldexp(1.0, -bits);
+        }
+        g.sink = d.sink ? 1 : 0;
+        g.reflect = (d.flags & IPIC_FLAG_REFLECT) ? 1 : 0;
+        {
+            const double L = d.cell_width[0];
+            g.dip_m = (float)(d.dipole_moment / (L * L * L));
+            for (int k = 0; k < 3; ++k) {
+                g.dip_o[k] = (float)(d.origin[k] / L);
+                g.dip_w[k] = (float)(d.cell_width[k] / L);
+            }
+        }
+        g.sink_r2 = (float)(d.sink_radius * d.sink_radius);
+
+        std::vector<int> jp;
+        if (build_jpairs(g, jp) != 0) {
+            delete ctx;
+            return fail(nullptr, IPIC_E_ARG, "tile/grid combination not supported");
+        }
+        hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            delete ctx;
+            return fail(nullptr, IPIC_E_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+        }
+        int rc = 0;
+        const size_t nc = (size_t)g.ncell;
+        rc = rc ? rc : dalloc(ctx, &ctx->E, nc);
+        rc = rc ? rc : dalloc(ctx, &ctx->Bc, nc);
+        rc = rc ? rc : dalloc(ctx, &ctx->Bn, nc);
+        rc = rc ? rc : dalloc(ctx, &ctx->J, nc);
+        rc = rc ? rc : dalloc(ctx, &ctx->jpart, (size_t)g.ntiles * g.nsl * 3 * g.NJ);
+        rc = rc ? rc : dalloc(ctx, &ctx->jover, nc * 3);
+        rc = rc ? rc : dalloc(ctx, &ctx->jflag, 1);
+        rc = rc ? rc : dalloc(ctx, &ctx->jpairs, jp.size());
+        rc = rc ? rc : dalloc(ctx, &ctx->removed, 1);
+        rc = rc ? rc : dalloc(ctx, &ctx->derr, 1);
+        rc = rc ? rc : dalloc(ctx, &ctx->tile_order, (size_t)g.ntiles);
+        rc = rc ? rc : dalloc(ctx, &ctx->red, (size_t)ctx->red_blocks * 5);
+        for (int s = 0; s < d.nspecies && !rc; ++s) {
+            Species& S = ctx->sp[s];
+            S.q = d.species[s].q;
+            S.qom = d.species[s].qom;
+            rc = rc ? rc : dalloc(ctx, &S.seg, nc + 1);
+            rc = rc ? rc : dalloc(ctx, &S.start, nc + 1);
+            rc = rc ? rc : dalloc(ctx, &S.stay, nc);
+            rc = rc ? rc : dalloc(ctx, &S.cnt_acc, nc);
+            rc = rc ? rc : dalloc(ctx, &S.stay_acc, nc);
+            rc = rc ? rc : dalloc(ctx, &S.arr, nc);
+            rc = rc ? rc : dalloc(ctx, &S.tile_total, (size_t)g.ntiles);
+            rc = rc ? rc : dalloc(ctx, &S.tile_base, (size_t)g.ntiles + 1);
+        }
+        if (!rc) {
+            e = hipMemcpyAsync(ctx->jpairs, jp.data(), jp.size() * sizeof(int), hipMemcpyHostToDevice, ctx->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+            if (e != hipSuccess) rc = fail(ctx, IPIC_E_HIP, std::string("upload: ") + hipGetErrorString(e));
+        }
+        if (!rc)
+            for (int k = 0; k < 8; ++k)
+                if (hipEventCreate(&ctx->ev[k]) != hipSuccess) rc = fail(ctx, IPIC_E_HIP, "hipEventCreate");
+        if (rc) {
+            std::string msg = ctx->err;
+            ipic_gpu_destroy(ctx);
+            g_last_error = msg;
+            return rc;
+        }
+        *out = ctx;
+        return 0;
+    });
 }
 
 int ipic_gpu_destroy(ipic_ctx* ctx) {
-    if (!ctx) return 0;
-    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-    for (void* p : ctx->allocs) (void)hipFree(p);
-    for (int k = 0; k < 8; ++k)
-        if (ctx->ev[k]) (void)hipEventDestroy(ctx->ev[k]);
-    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
-    delete ctx;
-    return 0;
+    return guarded(ctx, [&]() -> int {
+        if (!ctx) return 0;
+        if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+        for (void* p : ctx->allocs) (void)hipFree(p);
+        for (int k = 0; k < 8; ++k)
+            if (ctx->ev[k]) (void)hipEventDestroy(ctx->ev[k]);
+        if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+        return 0;
+    });
 }
 
 int ipic_gpu_upload_particles(ipic_ctx* ctx, int32_t species, int64_t n, const double* pos, const double* vel) {
-    if (int rc = check_ctx(ctx)) return rc;
-    if (species < 0 || species >= ctx->d.nspecies) return fail(ctx, IPIC_E_ARG, "species out of range");
-    if (n < 0 || (n > 0 && (!pos || !vel))) return fail(ctx, IPIC_E_ARG, "invalid particle arrays");
-    if (n >= (1LL << 31)) return fail(ctx, IPIC_E_RANGE, "at most 2^31-1 particles per species");
-    if (int rc = canonicalize(ctx)) return rc;
-    const Geo& g = ctx->g;
-    const int nc = g.ncell;
-    std::vector<int> cell((size_t)n);
-    std::vector<float> off((size_t)n * 3);
-    std::vector<int> count(nc + 1, 0);
-    for (int64_t i = 0; i < n; ++i) {
-        int ci[3];
-        for (int k = 0; k < 3; ++k) {
-            const double L = (double)g.n[k] * ctx->d.cell_width[k];
-            double x = pos[3 * i + k] - ctx->d.origin[k];
-            if (!(std::isfinite(x))) return fail(ctx, IPIC_E_ARG, "non-finite particle position");
-            x = std::fmod(x, L);
-            if (x < 0) x += L;
-            double qd = x / ctx->d.cell_width[k];
-            long long cc = (long long)std::floor(qd);
-            if (cc >= g.n[k]) cc = g.n[k] - 1;
-            if (cc < 0) cc = 0;
-            ci[k] = (int)cc;
-            float o = (float)(qd - (double)cc);
-            if (o < 0.0f) o = 0.0f;
-            if (o > 1.0f) o = 1.0f;
-            off[3 * i + k] = o;
+    return guarded(ctx, [&]() -> int {
+        if (int rc = check_ctx(ctx)) return rc;
+        if (species < 0 || species >= ctx->d.nspecies) return fail(ctx, IPIC_E_ARG, "species out of range");
+        if (n < 0 || (n > 0 && (!pos || !vel))) return fail(ctx, IPIC_E_ARG, "invalid particle arrays");
+        // the limit of alloc_species_buffers (one buffer resource spans the six arrays, < 4 GiB),
+        // checked before any host buffer is built
+        if (std::max<int64_t>(n, 1) * 24 >= (1LL << 32))
+            return fail(ctx, IPIC_E_RANGE, "a species holds at most 178956970 particles");
+        if (int rc = canonicalize(ctx)) return rc;
+        const Geo& g = ctx->g;
+        const int nc = g.ncell;
+        std::vector<int> cell((size_t)n);
+        std::vector<float> off((size_t)n * 3);
+        std::vector<int> count(nc + 1, 0);
+        for (int64_t i = 0; i < n; ++i) {
+            int ci[3];
+            for (int k = 0; k < 3; ++k) {
+                const double L = (double)g.n[k] * ctx->d.cell_width[k];
+                double x = pos[3 * i + k] - ctx->d.origin[k];
+                if (!(std::isfinite(x))) return fail(ctx, IPIC_E_ARG, "non-finite particle position");
+                x = std::fmod(x, L);
+                if (x < 0) x += L;
+                double qd = x / ctx->d.cell_width[k];
+                long long cc = (long long)std::floor(qd);
+                if (cc >= g.n[k]) cc = g.n[k] - 1;
+                if (cc < 0) cc = 0;
+                ci[k] = (int)cc;
+                float o = (float)(qd - (double)cc);
+                if (o < 0.0f) o = 0.0f;
+                if (o > 1.0f) o = 1.0f;
+                off[3 * i + k] = o;
+            }
+            cell[i] = host_cid(g, ci[0], ci[1], ci[2]);
+            count[cell[i] + 1]++;
         }
-        cell[i] = host_cid(g, ci[0], ci[1], ci[2]);
-        count[cell[i] + 1]++;
-    }
-    std::vector<int> start(nc + 1, 0);
-    for (int c = 0; c < nc; ++c) start[c + 1] = start[c] + count[c + 1];
-    std::vector<int> fillp(start.begin(), start.end() - 1);
-    const long long cap = std::max<long long>(n, 1);
-    std::vector<float> X((size_t)6 * cap, 0.0f);
-    std::vector<uint32_t> ids(ctx->track ? cap : 0);
-    for (int64_t i = 0; i < n; ++i) {
-        const int p = fillp[cell[i]]++;
-        X[p] = off[3 * i];
-        X[cap + p] = off[3 * i + 1];
-        X[2 * cap + p] = off[3 * i + 2];
-        X[3 * cap + p] = (float)vel[3 * i];
-        X[4 * cap + p] = (float)vel[3 * i + 1];
-        X[5 * cap + p] = (float)vel[3 * i + 2];
-        if (ctx->track) ids[p] = (uint32_t)i;
-    }
-    if (int rc = alloc_species_buffers(ctx, species, cap)) return rc;
-    Species& S = ctx->sp[species];
-    if (S.cap != cap) {
-        // keep the array stride of the allocation
-        std::vector<float> Y((size_t)6 * S.cap, 0.0f);
-        for (int k = 0; k < 6; ++k) std::memcpy(&Y[(size_t)k * S.cap], &X[(size_t)k * cap], (size_t)n * sizeof(float));
-        X.swap(Y);
-    }
-    HIPCK(ctx, hipMemcpyAsync(S.X[S.cur], X.data(), X.size() * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
-    if (ctx->track && n > 0)
-        HIPCK(ctx, hipMemcpyAsync(S.ids[S.cur], ids.data(), (size_t)n * sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream));
-    if (int rc = set_layout(ctx, species, start)) return rc;
-    ctx->j_valid = false;
-    return 0;
+        std::vector<int> start(nc + 1, 0);
+        for (int c = 0; c < nc; ++c) start[c + 1] = start[c] + count[c + 1];
+        std::vector<int> fillp(start.begin(), start.end() - 1);
+        const long long cap = std::max<long long>(n, 1);
+        std::vector<float> X((size_t)6 * cap, 0.0f);
+        std::vector<uint32_t> ids(ctx->track ? cap : 0);
+        for (int64_t i = 0; i < n; ++i) {
+            const int p = fillp[cell[i]]++;
+            X[p] = off[3 * i];
+            X[cap + p] = off[3 * i + 1];
+            X[2 * cap + p] = off[3 * i + 2];
+            X[3 * cap + p] = (float)vel[3 * i];
+            X[4 * cap + p] = (float)vel[3 * i + 1];
+            X[5 * cap + p] = (float)vel[3 * i + 2];
+            if (ctx->track) ids[p] = (uint32_t)i;
+        }
+        if (int rc = alloc_species_buffers(ctx, species, cap)) return rc;
+        Species& S = ctx->sp[species];
+        if (S.cap != cap) {
+            // keep the array stride of the allocation
+            std::vector<float> Y((size_t)6 * S.cap, 0.0f);
+            for (int k = 0; k < 6; ++k) std::memcpy(&Y[(size_t)k * S.cap], &X[(size_t)k * cap], (size_t)n * sizeof(float));
+            X.swap(Y);
+        }
+        HIPCK(ctx, hipMemcpyAsync(S.X[S.cur], X.data(), X.size() * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+        if (ctx->track && n > 0)
+            HIPCK(ctx, hipMemcpyAsync(S.ids[S.cur], ids.data(), (size_t)n * sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream));
+        if (int rc = set_layout(ctx, species, start)) return rc;
+        ctx->j_valid = false;
+        return 0;
+    });
 }
 
 int ipic_gpu_init_reference(ipic_ctx* ctx, int32_t species, const ipic_ref_distribution* dist, int64_t* counts) {
-    if (int rc = check_ctx(ctx)) return rc;
-    if (species < 0 || species >= ctx->d.nspecies) return fail(ctx, IPIC_E_ARG, "species out of range");
-    const int64_t nc = (int64_t)ctx->d.n[0] * ctx->d.n[1] * ctx->d.n[2];
-    std::vector<int64_t> cnt((size_t)nc);
-    if (int rc = ipic_ref_cell_counts(&ctx->d, dist, cnt.data())) return fail(ctx, rc, ipic_ref_last_error());
-    int64_t total = 0;
-    for (int64_t c = 0; c < nc; ++c) total += cnt[(size_t)c];
-    if (total >= (1LL << 31)) return fail(ctx, IPIC_E_RANGE, "at most 2^31-1 particles per species");
-    std::vector<double> pos, vel;
-    try {
-        pos.resize((size_t)total * 3);
-        vel.resize((size_t)total * 3);
-    } catch (const std::exception&) {
-        return fail(ctx, IPIC_E_RANGE, "host memory for the reference initialisation");
-    }
-    if (int rc = ipic_ref_particles(&ctx->d, dist, cnt.data(), pos.data(), vel.data()))
-        return fail(ctx, rc, ipic_ref_last_error());
-    if (counts) std::memcpy(counts, cnt.data(), (size_t)nc * sizeof(int64_t));
-    return ipic_gpu_upload_particles(ctx, species, total, pos.data(), vel.data());
+    return guarded(ctx, [&]() -> int {
+        if (int rc = check_ctx(ctx)) return rc;
+        if (species < 0 || species >= ctx->d.nspecies) return fail(ctx, IPIC_E_ARG, "species out of range");
+        const int64_t nc = (int64_t)ctx->d.n[0] * ctx->d.n[1] * ctx->d.n[2];
+        std::vector<int64_t> cnt((size_t)nc);
+        if (int rc = ipic_ref_cell_counts(&ctx->d, dist, cnt.data())) return fail(ctx, rc, ipic_ref_last_error());
+        int64_t total = 0;
+        for (int64_t c = 0; c < nc; ++c) total += cnt[(size_t)c];
+        if (total >= (1LL << 31)) return fail(ctx, IPIC_E_RANGE, "at most 2^31-1 particles per species");
+        std::vector<double> pos, vel;
+        try {
+            pos.resize((size_t)total * 3);
+            vel.resize((size_t)total * 3);
+        } catch (const std::exception&) {
+            return fail(ctx, IPIC_E_RANGE, "host memory for the reference initialisation");
+        }
+        if (int rc = ipic_ref_particles(&ctx->d, dist, cnt.data(), pos.data(), vel.data()))
+            return fail(ctx, rc, ipic_ref_last_error());
+        if (counts) std::memcpy(counts, cnt.data(), (size_t)nc * sizeof(int64_t));
+        return ipic_gpu_upload_particles(ctx, species, total, pos.data(), vel.data());
+    });
 }
 
 int ipic_gpu_init_particles(ipic_ctx* ctx, int32_t species, const int32_t* ppc, int32_t uniform_ppc,
                             const double vth[3], const double drift[3], uint64_t seed) {
-    if (int rc = check_ctx(ctx)) return rc;
-    if (species < 0 || species >= ctx->d.nspecies) return fail(ctx, IPIC_E_ARG, "species out of range");
-    if (!vth || !drift) return fail(ctx, IPIC_E_ARG, "null vth/drift");
-    if (!ppc && uniform_ppc < 0) return fail(ctx, IPIC_E_ARG, "negative particles per cell");
-    if (int rc = canonicalize(ctx)) return rc;
-    const Geo& g = ctx->g;
-    const int nc = g.ncell;
-    std::vector<int> start(nc + 1, 0);
-    long long total = 0;
-    for (int cid = 0; cid < nc; ++cid) {
-        int x, y, z;
-        cid_xyz(g, cid, x, y, z);
-        const int row = (x * g.n[1] + y) * g.n[2] + z;
-        const long long k = ppc ? ppc[row] : uniform_ppc;
-        if (k < 0) return fail(ctx, IPIC_E_ARG, "negative particles per cell");
-        total += k;
-        if (total >= (1LL << 31)) return fail(ctx, IPIC_E_RANGE, "at most 2^31-1 particles per species");
-        start[cid + 1] = (int)total;
-    }
-    if (int rc = alloc_species_buffers(ctx, species, std::max<long long>(total, 1))) return rc;
-    Species& S = ctx->sp[species];
-    if (int rc = set_layout(ctx, species, start)) return rc;
-    if (total > 0) {
-        const unsigned nb = (unsigned)((total + 255) / 256);
-        hipLaunchKernelGGL(k_init, dim3(nb), dim3(256), 0, ctx->stream, g, S.start, S.X[S.cur],
-                           ctx->track ? S.ids[S.cur] : nullptr, S.cap, total, seed, (int)species, vth[0], vth[1], vth[2],
-                           drift[0], drift[1], drift[2]);
-        HIPCK(ctx, hipGetLastError());
-    }
-    HIPCK(ctx, hipStreamSynchronize(ctx->stream));
-    ctx->j_valid = false;
-    return 0;
-}
-
-int ipic_gpu_upload_fields(ipic_ctx* ctx, const double* E, const double* Bc) {
-    if (int rc = check_ctx(ctx)) return rc;
-    const Geo& g = ctx->g;
-    const int nc = g.ncell;
-    const size_t n3 = (size_t)nc * 3;
-    // total fields: the given ones, or the current state for a NULL argument
-    std::vector<double> tE(n3), tB(n3);
-    if (!E || !Bc)
-        if (int rc = total_fields(ctx, E ? nullptr : tE.data(), Bc ? nullptr : tB.data())) return rc;
-    if (E) std::memcpy(tE.data(), E, n3 * sizeof(double));
-    if (Bc) std::memcpy(tB.data(), Bc, n3 * sizeof(double));
-    for (size_t i = 0; i < n3; ++i)
-        if (!std::isfinite(tE[i]) || !std::isfinite(tB[i])) return fail(ctx, IPIC_E_ARG, "non-finite field value");
-    ctx->bgE.swap(tE);
-    ctx->bgBc.swap(tB);
-    // uniform parts: mean over the grid
-    double e0[3] = {0, 0, 0}, b0[3] = {0, 0, 0};
-    for (int i = 0; i < nc; ++i)
-        for (int k = 0; k < 3; ++k) {
-            e0[k] += ctx->bgE[3 * i + k];
-            b0[k] += ctx->bgBc[3 * i + k];
-        }
-    for (int k = 0; k < 3; ++k) {
-        e0[k] /= nc;
-        b0[k] /= nc;
-        ctx->g.E0[k] = (float)e0[k];
-        ctx->g.B0[k] = (float)b0[k];
-    }
-    // non-uniform remainders relative to the FP32-rounded uniform parts
-    std::vector<double> rE(n3), rB(n3);
-    bool nuE = false, nuB = false;
-    for (int i = 0; i < nc; ++i)
-        for (int k = 0; k < 3; ++k) {
-            rE[3 * i + k] = ctx->bgE[3 * i + k] - (double)ctx->g.E0[k];
-            rB[3 * i + k] = ctx->bgBc[3 * i + k] - (double)ctx->g.B0[k];
-            nuE |= rE[3 * i + k] != 0.0;
-            nuB |= rB[3 * i + k] != 0.0;
-        }
-    if (ctx->d.flags & IPIC_FLAG_UNIFORM_BACKGROUND) {
-        // only E0, B0 stay static; the remainders start the evolving FP32 state
-        for (int i = 0; i < nc; ++i)
-            for (int k = 0; k < 3; ++k) {
-                ctx->bgE[3 * i + k] = (double)ctx->g.E0[k];
-                ctx->bgBc[3 * i + k] = (double)ctx->g.B0[k];
-            }
-        std::vector<float4> he(nc), hb(nc);
-        for (int i = 0; i < nc; ++i) {
-            he[i] = make_float4((float)rE[3 * i], (float)rE[3 * i + 1], (float)rE[3 * i + 2], 0.0f);
-            hb[i] = make_float4((float)rB[3 * i], (float)rB[3 * i + 1], (float)rB[3 * i + 2], 0.0f);
-        }
-        for (float4** a : {&ctx->Ebg, &ctx->Eg, &ctx->Bcbg, &ctx->Bnbg, &ctx->SE, &ctx->SB}) {
-            dfree(ctx, *a);
-            *a = nullptr;
-        }
-        HIPCK(ctx, hipMemcpyAsync(ctx->E, he.data(), nc * sizeof(float4), hipMemcpyHostToDevice, ctx->stream));
-        HIPCK(ctx, hipMemcpyAsync(ctx->Bc, hb.data(), nc * sizeof(float4), hipMemcpyHostToDevice, ctx->stream));
-        if (int rc = launch_c2n(ctx)) return rc;
-        HIPCK(ctx, hipStreamSynchronize(ctx->stream));
-        return 0;
-    }
-    // static sources of the background: SE = c^2 curl- Bc_bg (field.h:359-363),
-    // SB = curl+ E_bg (field.h:367-371), FP64
-    std::vector<double> sE(n3), sB(n3), rBn(n3);
-    const int* n = g.n;
-    const double* w = ctx->d.cell_width;
-    // curl factors of the field policy: leapfrog c^2 and 1, forward solver c and c
-    const bool fwd = (ctx->d.flags & IPIC_FLAG_FORWARD_SOLVER) != 0;
-    const double c2 = fwd ? ctx->d.c : ctx->d.c * ctx->d.c;
-    const double bF = fwd ? ctx->d.c : 1.0;
-    auto id = [&](int x, int y, int z) {
-        x = (x % n[0] + n[0]) % n[0];
-        y = (y % n[1] + n[1]) % n[1];
-        z = (z % n[2] + n[2]) % n[2];
-        return (size_t)((x * n[1] + y) * n[2] + z);
-    };
-    bool hasSE = false, hasSB = false;
-    const std::vector<double>& bB = ctx->bgBc;
-    const std::vector<double>& bE = ctx->bgE;
-    for (int x = 0; x < n[0]; ++x)
-        for (int y = 0; y < n[1]; ++y)
-            for (int z = 0; z < n[2]; ++z) {
-                const size_t i = id(x, y, z), mx = id(x - 1, y, z), my = id(x, y - 1, z), mz = id(x, y, z - 1);
-                const size_t px = id(x + 1, y, z), py = id(x, y + 1, z), pz = id(x, y, z + 1);
-                sE[3 * i + 0] = c2 * ((bB[3 * i + 2] - bB[3 * my + 2]) / w[1] - (bB[3 * i + 1] - bB[3 * mz + 1]) / w[2]);
-                sE[3 * i + 1] = c2 * ((bB[3 * i + 0] - bB[3 * mz + 0]) / w[2] - (bB[3 * i + 2] - bB[3 * mx + 2]) / w[0]);
-                sE[3 * i + 2] = c2 * ((bB[3 * i + 1] - bB[3 * mx + 1]) / w[0] - (bB[3 * i + 0] - bB[3 * my + 0]) / w[1]);
-                sB[3 * i + 0] = bF * ((bE[3 * py + 2] - bE[3 * i + 2]) / w[1] - (bE[3 * pz + 1] - bE[3 * i + 1]) / w[2]);
-                sB[3 * i + 1] = bF * ((bE[3 * pz + 0] - bE[3 * i + 0]) / w[2] - (bE[3 * px + 2] - bE[3 * i + 2]) / w[0]);
-                sB[3 * i + 2] = bF * ((bE[3 * px + 1] - bE[3 * i + 1]) / w[0] - (bE[3 * py + 0] - bE[3 * i + 0]) / w[1]);
-                for (int k = 0; k < 3; ++k) {
-                    hasSE |= sE[3 * i + k] != 0.0;
-                    hasSB |= sB[3 * i + k] != 0.0;
-                }
-                for (int k = 0; k < 3; ++k) {
-                    double acc = 0;
-                    for (int a = 0; a < 2; ++a)
-                        for (int b = 0; b < 2; ++b)
-                            for (int c = 0; c < 2; ++c) acc += rB[3 * id(x - a, y - b, z - c) + k];
-                    rBn[3 * i + k] = 0.125 * acc;
-                }
-            }
-    auto put = [&](float4** dst, bool need, const std::vector<double>& src) -> int {
-        if (!need) {
-            dfree(ctx, *dst);
-            *dst = nullptr;
-            return 0;
-        }
-        if (!*dst)
-            if (int rc = dalloc(ctx, dst, (size_t)nc)) return rc;
-        std::vector<float4> h(nc);
-        for (int i = 0; i < nc; ++i)
-            h[i] = make_float4((float)src[3 * i], (float)src[3 * i + 1], (float)src[3 * i + 2], 0.0f);
-        HIPCK(ctx, hipMemcpyAsync(*dst, h.data(), nc * sizeof(float4), hipMemcpyHostToDevice, ctx->stream));
-        HIPCK(ctx, hipStreamSynchronize(ctx->stream));
-        return 0;
-    };
-    if (int rc = put(&ctx->Ebg, nuE, rE)) return rc;
-    if (int rc = put(&ctx->Eg, nuE, rE)) return rc;  // gather E = dE + Ebg with dE = 0
-    if (int rc = put(&ctx->Bcbg, nuB, rB)) return rc;
-    if (int rc = put(&ctx->Bnbg, nuB, rBn)) return rc;
-    if (int rc = put(&ctx->SE, hasSE, sE)) return rc;
-    if (int rc = put(&ctx->SB, hasSB, sB)) return rc;
-    // perturbations restart from zero; gather B = interpC2N(Bcbg)
-    HIPCK(ctx, hipMemsetAsync(ctx->E, 0, nc * sizeof(float4), ctx->stream));
-    HIPCK(ctx, hipMemsetAsync(ctx->Bc, 0, nc * sizeof(float4), ctx->stream));
-    HIPCK(ctx, hipMemsetAsync(ctx->Bn, 0, nc * sizeof(float4), ctx->stream));
-    if (nuB) HIPCK(ctx, hipMemcpyAsync(ctx->Bn, ctx->Bnbg, nc * sizeof(float4), hipMemcpyDeviceToDevice, ctx->stream));
-    HIPCK(ctx, hipStreamSynchronize(ctx->stream));
-    return 0;
-}
-
-int ipic_gpu_step(ipic_ctx* ctx, int64_t nsteps, ipic_step_stats* stats) {
-    if (int rc = check_ctx(ctx)) return rc;
-    if (nsteps < 0) return fail(ctx, IPIC_E_ARG, "negative step count");
-    ipic_step_stats st{};
-    HIPCK(ctx, hipMemsetAsync(ctx->removed, 0, sizeof(unsigned long long), ctx->stream));
-    HIPCK(ctx, hipStreamSynchronize(ctx->stream));
-    auto t0 = std::chrono::steady_clock::now();
-    auto t1 = t0;
-    for (int64_t s = 0; s < nsteps; ++s) {
-        if (int rc = field_phase(ctx, true, &st)) return rc;
-        if (int rc = particle_phase(ctx, &st)) return rc;
-        if (s == 0) {
-            HIPCK(ctx, hipStreamSynchronize(ctx->stream));
-            t1 = std::chrono::steady_clock::now();
-        }
-    }
-    HIPCK(ctx, hipStreamSynchronize(ctx->stream));
-    auto t2 = std::chrono::steady_clock::now();
-    if (int rc = check_device_flag(ctx)) return rc;
-    unsigned long long removed = 0;
-    HIPCK(ctx, hipMemcpy(&removed, ctx->removed, sizeof(removed), hipMemcpyDeviceToHost));
-    st.first_step_s = std::chrono::duration<double>(t1 - t0).count();
-    st.remaining_steps_s = std::chrono::duration<double>(t2 - t1).count();
-    st.steps = nsteps;
-    st.removed = (int64_t)removed;
-    long long tot = 0;
-    for (int s = 0; s < ctx->d.nspecies; ++s) {
-        long long t = 0;
-        if (int rc = read_total(ctx, s, &t)) return rc;
-        tot += t;
-    }
-    st.particles = tot;
-    if (stats) *stats = st;
-    return 0;
-}
-
-int ipic_gpu_field_step(ipic_ctx* ctx, int32_t use_operator, ipic_step_stats* stats) {
-    if (int rc = check_ctx(ctx)) return rc;
-    if (use_operator && !ctx->op_on) return fail(ctx, IPIC_E_STATE, "no field operator set");
-    ipic_step_stats st{};
-    if (int rc = field_phase(ctx, use_operator != 0, &st)) return rc;
-    HIPCK(ctx, hipStreamSynchronize(ctx->stream));
-    if (stats) {
-        stats->ms_field += st.ms_field;
-        stats->ms_deposit += st.ms_deposit;
-        stats->operator_steps += st.operator_steps;
-        stats->fallback_steps += st.fallback_steps;
-        stats->audits += st.audits;
-        stats->max_audit_error = std::max(stats->max_audit_error, st.max_audit_error);
-        stats->max_audit_error_increment = std::max(stats->max_audit_error_increment, st.max_audit_error_increment);
-        stats->max_divb_residual = std::max(stats->max_divb_residual, st.max_divb_residual);
-        stats->max_energy_residual = std::max(stats->max_energy_residual, st.max_energy_residual);
-    }
-    return 0;
-}
-
-int ipic_gpu_particle_step(ipic_ctx* ctx, ipic_step_stats* stats) {
-    if (int rc = check_ctx(ctx)) return rc;
-    ipic_step_stats st{};
-    HIPCK(ctx, hipMemsetAsync(ctx->removed, 0, sizeof(unsigned long long), ctx->stream));
-    if (int rc = particle_phase(ctx, &st)) return rc;
-    HIPCK(ctx, hipStreamSynchronize(ctx->stream));
-    if (int rc = check_device_flag(ctx)) return rc;
-    if (stats) {
-        unsigned long long removed = 0;
-        HIPCK(ctx, hipMemcpy(&removed, ctx->removed, sizeof(removed), hipMemcpyDeviceToHost));
-        stats->removed += (int64_t)removed;
-        stats->ms_push += st.ms_push;
-        stats->ms_scan += st.ms_scan;
-        stats->steps += 1;
-    }
-    return 0;
-}
-
-int ipic_gpu_set_field_operator(ipic_ctx* ctx, const ipic_field_operator* op) {
-    if (int rc = check_ctx(ctx)) return rc;
-    if (!op) {
-        ctx->op_on = false;
-        return 0;
-    }
-    if (!op->params) return fail(ctx, IPIC_E_ARG, "null operator parameters");
-    if (ctx->d.flags & IPIC_FLAG_FORWARD_SOLVER)
-        return fail(ctx, IPIC_E_STATE, "the field operator stands in for leapfrog substeps");
-    if (op->substeps < 1 || op->substeps != ctx->d.field_substeps)
-        return fail(ctx, IPIC_E_ARG, "operator substeps differ from the context's field_substeps");
-    if (!(op->in_E > 0) || !(op->in_B > 0) || !(op->in_J > 0) || !std::isfinite(op->out_E) || !std::isfinite(op->out_P))
-        return fail(ctx, IPIC_E_ARG, "operator scales must be positive and finite");
-    if (op->audit_every < 0) return fail(ctx, IPIC_E_ARG, "negative audit interval");
-    const Geo& g = ctx->g;
-    const int32_t n3[3] = {g.n[0], g.n[1], g.n[2]};
-    const long long wsb = ipic_nop_workspace_bytes(n3, 0);
-    if (wsb < 0) return nop_fail(ctx, IPIC_E_ARG);
-    const long long np = ipic_nop_param_count();
-    const size_t nc = (size_t)g.ncell;
-    if (!ctx->op_params) {
-        if (int rc = dalloc(ctx, &ctx->op_params, (size_t)np)) return rc;
-        if (int rc = dalloc(ctx, &ctx->op_wf, (size_t)np)) return rc;
-        if (int rc = dalloc(ctx, &ctx->op_ws, (size_t)wsb)) return rc;
-        if (int rc = dalloc(ctx, &ctx->op_u, nc * 16)) return rc;
-        if (int rc = dalloc(ctx, &ctx->op_out, nc * 16)) return rc;
-        if (int rc = dalloc(ctx, &ctx->op_psi, nc)) return rc;
-        if (int rc = dalloc(ctx, &ctx->Eb, nc)) return rc;
-        if (int rc = dalloc(ctx, &ctx->Bb, nc)) return rc;
-        if (int rc = dalloc(ctx, &ctx->Eo, nc)) return rc;
-        if (int rc = dalloc(ctx, &ctx->Bo, nc)) return rc;
-    }
-    for (long long i = 0; i < np; ++i)
-        if (!std::isfinite(op->params[i])) return fail(ctx, IPIC_E_ARG, "non-finite operator parameter");
-    HIPCK(ctx, hipMemcpyAsync(ctx->op_params, op->params, (size_t)np * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
-    if (int rc = ipic_nop_prepare(ctx->op_params, ctx->op_wf, nullptr, ctx->stream)) return nop_fail(ctx, rc);
-    HIPCK(ctx, hipStreamSynchronize(ctx->stream));
-    ctx->op = *op;
-    ctx->op.params = nullptr;
-    ctx->op_scales = ipic_nop_scales{op->in_E, op->in_B, op->in_J, op->out_E, op->out_P};
-    ctx->op_count = 0;
-    ctx->op_on = true;
-    return 0;
-}
-
-int ipic_gpu_field_state(ipic_ctx* ctx, ipic_field_state* out) {
-    if (int rc = check_ctx(ctx)) return rc;
-    if (!out) return fail(ctx, IPIC_E_ARG, "null output");
-    if (!ctx->j_valid) {
-        if (int rc = launch_particles(ctx, MODE_DEPOSIT)) return rc;
-        ctx->j_valid = true;
-    }
-    if (int rc = launch_jsum(ctx)) return rc;
-    HIPCK(ctx, hipStreamSynchronize(ctx->stream));
-    out->E = (float*)ctx->E;
-    out->Bc = (float*)ctx->Bc;
-    out->J = (float*)ctx->J;
-    for (int k = 0; k < 3; ++k) {
-        out->E0[k] = ctx->g.E0[k];
-        out->B0[k] = ctx->g.B0[k];
-    }
-    out->uniform_background = (ctx->Ebg || ctx->Bcbg || ctx->SE || ctx->SB) ? 0 : 1;
-    return 0;
-}
-
-int ipic_gpu_num_particles(ipic_ctx* ctx, int32_t species, int64_t* n) {
-    if (int rc = check_ctx(ctx)) return rc;
-    if (!n) return fail(ctx, IPIC_E_ARG, "null output");
-    if (species >= ctx->d.nspecies) return fail(ctx, IPIC_E_ARG, "species out of range");
-    if (int rc = canonicalize(ctx)) return rc;
-    long long tot = 0;
-    for (int s = 0; s < ctx->d.nspecies; ++s) {
-        if (species >= 0 && s != species) continue;
-        long long t = 0;
-        if (int rc = read_total(ctx, s, &t)) return rc;
-        tot += t;
-    }
-    *n = tot;
-    return 0;
-}
-
-int ipic_gpu_cell_counts(ipic_ctx* ctx, int32_t species, int64_t* counts) {
-    if (int rc = check_ctx(ctx)) return rc;
-    if (!counts) return fail(ctx, IPIC_E_ARG, "null output");
-    if (species >= ctx->d.nspecies) return fail(ctx, IPIC_E_ARG, "species out of range");
-    if (int rc = canonicalize(ctx)) return rc;
-    const Geo& g = ctx->g;
-    const int nc = g.ncell;
-    std::vector<int> st(nc + 1);
-    std::memset(counts, 0, (size_t)nc * sizeof(int64_t));
-    for (int s = 0; s < ctx->d.nspecies; ++s) {
-        if (species >= 0 && s != species) continue;
-        HIPCK(ctx, hipMemcpy(st.data(), ctx->sp[s].start, (nc + 1) * sizeof(int), hipMemcpyDeviceToHost));
+    return guarded(ctx, [&]() -> int {
+        if (int rc = check_ctx(ctx)) return rc;
+        if (species < 0 || species >= ctx->d.nspecies) return fail(ctx, IPIC_E_ARG, "species out of range");
+        if (!vth || !drift) return fail(ctx, IPIC_E_ARG, "null vth/drift");
+        if (!ppc && uniform_ppc < 0) return fail(ctx, IPIC_E_ARG, "negative particles per cell");
+        if (int rc = canonicalize(ctx)) return rc;
+        const Geo& g = ctx->g;
+        const int nc = g.ncell;
+        std::vector<int> start(nc + 1, 0);
+        long long total = 0;
         for (int cid = 0; cid < nc; ++cid) {
             int x, y, z;
             cid_xyz(g, cid, x, y, z);
-            counts[(x * g.n[1] + y) * g.n[2] + z] += st[cid + 1] - st[cid];
+            const int row = (x * g.n[1] + y) * g.n[2] + z;
+            const long long k = ppc ? ppc[row] : uniform_ppc;
+            if (k < 0) return fail(ctx, IPIC_E_ARG, "negative particles per cell");
+            total += k;
+            if (total >= (1LL << 31)) return fail(ctx, IPIC_E_RANGE, "at most 2^31-1 particles per species");
+            start[cid + 1] = (int)total;
         }
-    }
-    return 0;
+        if (int rc = alloc_species_buffers(ctx, species, std::max<long long>(total, 1))) return rc;
+        Species& S = ctx->sp[species];
+        if (int rc = set_layout(ctx, species, start)) return rc;
+        if (total > 0) {
+            const unsigned nb = (unsigned)((total + 255) / 256);
+            hipLaunchKernelGGL(k_init, dim3(nb), dim3(256), 0, ctx->stream, g, S.start, S.X[S.cur],
+                               ctx->track ? S.ids[S.cur] : nullptr, S.cap, total, seed, (int)species, vth[0], vth[1], vth[2],
+                               drift[0], drift[1], drift[2]);
+            HIPCK(ctx, hipGetLastError());
+        }
+        HIPCK(ctx, hipStreamSynchronize(ctx->stream));
+        ctx->j_valid = false;
+        return 0;
+    });
 }
 
-int ipic_gpu_download_particles(ipic_ctx* ctx, int32_t species, double* pos, double* vel, int64_t* cell,
-                                uint32_t* ids) {
-    if (int rc = check_ctx(ctx)) return rc;
-    if (species < 0 || species >= ctx->d.nspecies) return fail(ctx, IPIC_E_ARG, "species out of range");
-    if (ids && !ctx->track) return fail(ctx, IPIC_E_STATE, "ids need IPIC_FLAG_TRACK_IDS");
-    if (int rc = canonicalize(ctx)) return rc;
-    const Geo& g = ctx->g;
-    const int nc = g.ncell;
-    Species& S = ctx->sp[species];
-    std::vector<int> st(nc + 1);
-    HIPCK(ctx, hipMemcpy(st.data(), S.start, (nc + 1) * sizeof(int), hipMemcpyDeviceToHost));
-    const long long n = st[nc];
-    std::vector<float> X((size_t)6 * std::max<long long>(n, 1));
-    for (int k = 0; k < 6 && n > 0; ++k)
-        HIPCK(ctx, hipMemcpy(&X[(size_t)k * n], S.X[S.cur] + (size_t)k * S.cap, (size_t)n * sizeof(float),
-                             hipMemcpyDeviceToHost));
-    std::vector<uint32_t> hid(ids ? std::max<long long>(n, 1) : 0);
-    if (ids && n > 0) HIPCK(ctx, hipMemcpy(hid.data(), S.ids[S.cur], (size_t)n * sizeof(uint32_t), hipMemcpyDeviceToHost));
-    long long o = 0;
-    for (int x = 0; x < g.n[0]; ++x)
-        for (int y = 0; y < g.n[1]; ++y)
-            for (int z = 0; z < g.n[2]; ++z) {
-                const int cid = host_cid(g, x, y, z);
-                const int cc[3] = {x, y, z};
-                for (int p = st[cid]; p < st[cid + 1]; ++p, ++o) {
-                    for (int k = 0; k < 3; ++k) {
-                        if (pos) pos[3 * o + k] = ctx->d.origin[k] + ((double)cc[k] + (double)X[(size_t)k * n + p]) * ctx->d.cell_width[k];
-                        if (vel) vel[3 * o + k] = (double)X[(size_t)(3 + k) * n + p];
-                    }
-                    if (cell) cell[o] = ((long long)x * g.n[1] + y) * g.n[2] + z;
-                    if (ids) ids[o] = hid[p];
-                }
+int ipic_gpu_upload_fields(ipic_ctx* ctx, const double* E, const double* Bc) {
+    return guarded(ctx, [&]() -> int {
+        if (int rc = check_ctx(ctx)) return rc;
+        const Geo& g = ctx->g;
+        const int nc = g.ncell;
+        const size_t n3 = (size_t)nc * 3;
+        // total fields: the given ones, or the current state for a NULL argument
+        std::vector<double> tE(n3), tB(n3);
+        if (!E || !Bc)
+            if (int rc = total_fields(ctx, E ? nullptr : tE.data(), Bc ? nullptr : tB.data())) return rc;
+        if (E) std::memcpy(tE.data(), E, n3 * sizeof(double));
+        if (Bc) std::memcpy(tB.data(), Bc, n3 * sizeof(double));
+        for (size_t i = 0; i < n3; ++i)
+            if (!std::isfinite(tE[i]) || !std::isfinite(tB[i])) return fail(ctx, IPIC_E_ARG, "non-finite field value");
+        ctx->bgE.swap(tE);
+        ctx->bgBc.swap(tB);
+        // uniform parts: mean over the grid
+        double e0[3] = {0, 0, 0}, b0[3] = {0, 0, 0};
+        for (int i = 0; i < nc; ++i)
+            for (int k = 0; k < 3; ++k) {
+                e0[k] += ctx->bgE[3 * i + k];
+                b0[k] += ctx->bgBc[3 * i + k];
             }
-    return 0;
-}
-
-int ipic_gpu_download_fields(ipic_ctx* ctx, double* E, double* Bc, double* Bn, double* J) {
-    if (int rc = check_ctx(ctx)) return rc;
-    const Geo& g = ctx->g;
-    const int nc = g.ncell;
-    std::vector<double> tB;
-    if (Bn && !Bc) tB.resize((size_t)nc * 3);
-    double* bc = Bc ? Bc : (Bn ? tB.data() : nullptr);
-    if (int rc = total_fields(ctx, E, bc)) return rc;
-    if (Bn) {
-        // interpC2N (field.h:234-247) of the total Bc, FP64
+        for (int k = 0; k < 3; ++k) {
+            e0[k] /= nc;
+            b0[k] /= nc;
+            ctx->g.E0[k] = (float)e0[k];
+            ctx->g.B0[k] = (float)b0[k];
+        }
+        // non-uniform remainders relative to the FP32-rounded uniform parts
+        std::vector<double> rE(n3), rB(n3);
+        bool nuE = false, nuB = false;
+        for (int i = 0; i < nc; ++i)
+            for (int k = 0; k < 3; ++k) {
+                rE[3 * i + k] = ctx->bgE[3 * i + k] - (double)ctx->g.E0[k];
+                rB[3 * i + k] = ctx->bgBc[3 * i + k] - (double)ctx->g.B0[k];
+                nuE |= rE[3 * i + k] != 0.0;
+                nuB |= rB[3 * i + k] != 0.0;
+            }
+        if (ctx->d.flags & IPIC_FLAG_UNIFORM_BACKGROUND) {
+            // only E0, B0 stay static; the remainders start the evolving FP32 state
+            for (int i = 0; i < nc; ++i)
+                for (int k = 0; k < 3; ++k) {
+                    ctx->bgE[3 * i + k] = (double)ctx->g.E0[k];
+                    ctx->bgBc[3 * i + k] = (double)ctx->g.B0[k];
+                }
+            std::vector<float4> he(nc), hb(nc);
+            for (int i = 0; i < nc; ++i) {
+                he[i] = make_float4((float)rE[3 * i], (float)rE[3 * i + 1], (float)rE[3 * i + 2], 0.0f);
+                hb[i] = make_float4((float)rB[3 * i], (float)rB[3 * i + 1], (float)rB[3 * i + 2], 0.0f);
+            }
+            for (float4** a : {&ctx->Ebg, &ctx->Eg, &ctx->Bcbg, &ctx->Bnbg, &ctx->SE, &ctx->SB}) {
+                dfree(ctx, *a);
+                *a = nullptr;
+            }
+            HIPCK(ctx, hipMemcpyAsync(ctx->E, he.data(), nc * sizeof(float4), hipMemcpyHostToDevice, ctx->stream));
+            HIPCK(ctx, hipMemcpyAsync(ctx->Bc, hb.data(), nc * sizeof(float4), hipMemcpyHostToDevice, ctx->stream));
+            if (int rc = launch_c2n(ctx)) return rc;
+            HIPCK(ctx, hipStreamSynchronize(ctx->stream));
+            return 0;
+        }
+        // static sources of the background: SE = c^2 curl- Bc_bg (field.h:359-363),
+        // SB = curl+ E_bg (field.h:367-371), FP64
+        std::vector<double> sE(n3), sB(n3), rBn(n3);
         const int* n = g.n;
+        const double* w = ctx->d.cell_width;
+        // curl factors of the field policy: leapfrog c^2 and 1, forward solver c and c
+        const bool fwd = (ctx->d.flags & IPIC_FLAG_FORWARD_SOLVER) != 0;
+        const double c2 = fwd ? ctx->d.c : ctx->d.c * ctx->d.c;
+        const double bF = fwd ? ctx->d.c : 1.0;
+        auto id = [&](int x, int y, int z) {
+            x = (x % n[0] + n[0]) % n[0];
+            y = (y % n[1] + n[1]) % n[1];
+            z = (z % n[2] + n[2]) % n[2];
+            return (size_t)((x * n[1] + y) * n[2] + z);
+        };
+        bool hasSE = false, hasSB = false;
+        const std::vector<double>& bB = ctx->bgBc;
+        const std::vector<double>& bE = ctx->bgE;
         for (int x = 0; x < n[0]; ++x)
             for (int y = 0; y < n[1]; ++y)
                 for (int z = 0; z < n[2]; ++z) {
-                    const size_t i = (size_t)((x * n[1] + y) * n[2] + z);
+                    const size_t i = id(x, y, z), mx = id(x - 1, y, z), my = id(x, y - 1, z), mz = id(x, y, z - 1);
+                    const size_t px = id(x + 1, y, z), py = id(x, y + 1, z), pz = id(x, y, z + 1);
+                    sE[3 * i + 0] = c2 * ((bB[3 * i + 2] - bB[3 * my + 2]) / w[1] - (bB[3 * i + 1] - bB[3 * mz + 1]) / w[2]);
+                    sE[3 * i + 1] = c2 * ((bB[3 * i + 0] - bB[3 * mz + 0]) / w[2] - (bB[3 * i + 2] - bB[3 * mx + 2]) / w[0]);
+                    sE[3 * i + 2] = c2 * ((bB[3 * i + 1] - bB[3 * mx + 1]) / w[0] - (bB[3 * i + 0] - bB[3 * my + 0]) / w[1]);
+                    sB[3 * i + 0] = bF * ((bE[3 * py + 2] - bE[3 * i + 2]) / w[1] - (bE[3 * pz + 1] - bE[3 * i + 1]) / w[2]);
+                    sB[3 * i + 1] = bF * ((bE[3 * pz + 0] - bE[3 * i + 0]) / w[2] - (bE[3 * px + 2] - bE[3 * i + 2]) / w[0]);
+                    sB[3 * i + 2] = bF * ((bE[3 * px + 1] - bE[3 * i + 1]) / w[0] - (bE[3 * py + 0] - bE[3 * i + 0]) / w[1]);
+                    for (int k = 0; k < 3; ++k) {
+                        hasSE |= sE[3 * i + k] != 0.0;
+                        hasSB |= sB[3 * i + k] != 0.0;
+                    }
                     for (int k = 0; k < 3; ++k) {
                         double acc = 0;
                         for (int a = 0; a < 2; ++a)
                             for (int b = 0; b < 2; ++b)
-                                for (int c = 0; c < 2; ++c) {
-                                    const int xx = (x - a + n[0]) % n[0], yy = (y - b + n[1]) % n[1],
-                                              zz = (z - c + n[2]) % n[2];
-                                    acc += bc[3 * ((xx * n[1] + yy) * n[2] + zz) + k];
-                                }
-                        Bn[3 * i + k] = 0.125 * acc;
+                                for (int c = 0; c < 2; ++c) acc += rB[3 * id(x - a, y - b, z - c) + k];
+                        rBn[3 * i + k] = 0.125 * acc;
                     }
                 }
-    }
-    if (J) {
-        std::vector<float4> h(nc);
-        HIPCK(ctx, hipMemcpy(h.data(), ctx->J, nc * sizeof(float4), hipMemcpyDeviceToHost));
-        for (int i = 0; i < nc; ++i) {
-            J[3 * i] = h[i].x;
-            J[3 * i + 1] = h[i].y;
-            J[3 * i + 2] = h[i].z;
+        auto put = [&](float4** dst, bool need, const std::vector<double>& src) -> int {
+            if (!need) {
+                dfree(ctx, *dst);
+                *dst = nullptr;
+                return 0;
+            }
+            if (!*dst)
+                if (int rc = dalloc(ctx, dst, (size_t)nc)) return rc;
+            std::vector<float4> h(nc);
+            for (int i = 0; i < nc; ++i)
+                h[i] = make_float4((float)src[3 * i], (float)src[3 * i + 1], (float)src[3 * i + 2], 0.0f);
+            HIPCK(ctx, hipMemcpyAsync(*dst, h.data(), nc * sizeof(float4), hipMemcpyHostToDevice, ctx->stream));
+            HIPCK(ctx, hipStreamSynchronize(ctx->stream));
+            return 0;
+        };
+        if (int rc = put(&ctx->Ebg, nuE, rE)) return rc;
+        if (int rc = put(&ctx->Eg, nuE, rE)) return rc;  // gather E = dE + Ebg with dE = 0
+        if (int rc = put(&ctx->Bcbg, nuB, rB)) return rc;
+        if (int rc = put(&ctx->Bnbg, nuB, rBn)) return rc;
+        if (int rc = put(&ctx->SE, hasSE, sE)) return rc;
+        if (int rc = put(&ctx->SB, hasSB, sB)) return rc;
+        // perturbations restart from zero; gather B = interpC2N(Bcbg)
+        HIPCK(ctx, hipMemsetAsync(ctx->E, 0, nc * sizeof(float4), ctx->stream));
+        HIPCK(ctx, hipMemsetAsync(ctx->Bc, 0, nc * sizeof(float4), ctx->stream));
+        HIPCK(ctx, hipMemsetAsync(ctx->Bn, 0, nc * sizeof(float4), ctx->stream));
+        if (nuB) HIPCK(ctx, hipMemcpyAsync(ctx->Bn, ctx->Bnbg, nc * sizeof(float4), hipMemcpyDeviceToDevice, ctx->stream));
+        HIPCK(ctx, hipStreamSynchronize(ctx->stream));
+        return 0;
+    });
+}
+
+int ipic_gpu_init_fields(ipic_ctx* ctx, const ipic_field_init* init, double energies[2]) {
+    return guarded(ctx, [&]() -> int {
+        if (int rc = check_ctx(ctx)) return rc;
+        if (!init) return fail(ctx, IPIC_E_ARG, "null field initialisation");
+        const size_t n3 = (size_t)ctx->g.ncell * 3;
+        std::vector<double> E(n3), Bc(n3);
+        if (int rc = ipic_field_init_host(&ctx->d, init, E.data(), nullptr, Bc.data(), energies))
+            return fail(ctx, rc, ipic_field_init_last_error());
+        return ipic_gpu_upload_fields(ctx, E.data(), Bc.data());
+    });
+}
+
+int ipic_gpu_step(ipic_ctx* ctx, int64_t nsteps, ipic_step_stats* stats) {
+    return guarded(ctx, [&]() -> int {
+        if (int rc = check_ctx(ctx)) return rc;
+        if (nsteps < 0) return fail(ctx, IPIC_E_ARG, "negative step count");
+        ipic_step_stats st{};
+        HIPCK(ctx, hipMemsetAsync(ctx->removed, 0, sizeof(unsigned long long), ctx->stream));
+        HIPCK(ctx, hipStreamSynchronize(ctx->stream));
+        auto t0 = std::chrono::steady_clock::now();
+        auto t1 = t0;
+        for (int64_t s = 0; s < nsteps; ++s) {
+            if (int rc = field_phase(ctx, true, &st)) return rc;
+            if (int rc = particle_phase(ctx, &st)) return rc;
+            if (s == 0) {
+                HIPCK(ctx, hipStreamSynchronize(ctx->stream));
+                t1 = std::chrono::steady_clock::now();
+            }
         }
-    }
-    return 0;
+        HIPCK(ctx, hipStreamSynchronize(ctx->stream));
+        auto t2 = std::chrono::steady_clock::now();
+        if (int rc = check_device_flag(ctx)) return rc;
+        unsigned long long removed = 0;
+        HIPCK(ctx, hipMemcpy(&removed, ctx->removed, sizeof(removed), hipMemcpyDeviceToHost));
+        st.first_step_s = std::chrono::duration<double>(t1 - t0).count();
+        st.remaining_steps_s = std::chrono::duration<double>(t2 - t1).count();
+        st.steps = nsteps;
+        st.removed = (int64_t)removed;
+        long long tot = 0;
+        for (int s = 0; s < ctx->d.nspecies; ++s) {
+            long long t = 0;
+            if (int rc = read_total(ctx, s, &t)) return rc;
+            tot += t;
+        }
+        st.particles = tot;
+        if (stats) *stats = st;
+        return 0;
+    });
+}
+
+int ipic_gpu_field_step(ipic_ctx* ctx, int32_t use_operator, ipic_step_stats* stats) {
+    return guarded(ctx, [&]() -> int {
+        if (int rc = check_ctx(ctx)) return rc;
+        if (use_operator && !ctx->op_on) return fail(ctx, IPIC_E_STATE, "no field operator set");
+        ipic_step_stats st{};
+        if (int rc = field_phase(ctx, use_operator != 0, &st)) return rc;
+        HIPCK(ctx, hipStreamSynchronize(ctx->stream));
+        if (stats) {
+            stats->ms_field += st.ms_field;
+            stats->ms_deposit += st.ms_deposit;
+            stats->operator_steps += st.operator_steps;
+            stats->fallback_steps += st.fallback_steps;
+            stats->audits += st.audits;
+            stats->max_audit_error = std::max(stats->max_audit_error, st.max_audit_error);
+            stats->max_audit_error_increment = std::max(stats->max_audit_error_increment, st.max_audit_error_increment);
+            stats->max_divb_residual = std::max(stats->max_divb_residual, st.max_divb_residual);
+            stats->max_energy_residual = std::max(stats->max_energy_residual, st.max_energy_residual);
+        }
+        return 0;
+    });
+}
+
+int ipic_gpu_particle_step(ipic_ctx* ctx, ipic_step_stats* stats) {
+    return guarded(ctx, [&]() -> int {
+        if (int rc = check_ctx(ctx)) return rc;
+        ipic_step_stats st{};
+        HIPCK(ctx, hipMemsetAsync(ctx->removed, 0, sizeof(unsigned long long), ctx->stream));
+        if (int rc = particle_phase(ctx, &st)) return rc;
+        HIPCK(ctx, hipStreamSynchronize(ctx->stream));
+        if (int rc = check_device_flag(ctx)) return rc;
+        if (stats) {
+            unsigned long long removed = 0;
+            HIPCK(ctx, hipMemcpy(&removed, ctx->removed, sizeof(removed), hipMemcpyDeviceToHost));
+            stats->removed += (int64_t)removed;
+            stats->ms_push += st.ms_push;
+            stats->ms_scan += st.ms_scan;
+            stats->steps += 1;
+        }
+        return 0;
+    });
+}
+
+int ipic_gpu_set_field_operator(ipic_ctx* ctx, const ipic_field_operator* op) {
+    return guarded(ctx, [&]() -> int {
+        if (int rc = check_ctx(ctx)) return rc;
+        if (!op) {
+            ctx->op_on = false;
+            return 0;
+        }
+        if (!op->params) return fail(ctx, IPIC_E_ARG, "null operator parameters");
+        if (ctx->d.flags & IPIC_FLAG_FORWARD_SOLVER)
+            return fail(ctx, IPIC_E_STATE, "the field operator stands in for leapfrog substeps");
+        if (op->substeps < 1 || op->substeps != ctx->d.field_substeps)
+            return fail(ctx, IPIC_E_ARG, "operator substeps differ from the context's field_substeps");
+        if (!(op->in_E > 0) || !(op->in_B > 0) || !(op->in_J > 0) || !std::isfinite(op->out_E) || !std::isfinite(op->out_P))
+            return fail(ctx, IPIC_E_ARG, "operator scales must be positive and finite");
+        if (op->audit_every < 0) return fail(ctx, IPIC_E_ARG, "negative audit interval");
+        const Geo& g = ctx->g;
+        const int32_t n3[3] = {g.n[0], g.n[1], g.n[2]};
+        const long long wsb = ipic_nop_workspace_bytes(n3, 0);
+        if (wsb < 0) return nop_fail(ctx, IPIC_E_ARG);
+        const long long np = ipic_nop_param_count();
+        const size_t nc = (size_t)g.ncell;
+        if (!ctx->op_params) {
+            if (int rc = dalloc(ctx, &ctx->op_params, (size_t)np)) return rc;
+            if (int rc = dalloc(ctx, &ctx->op_wf, (size_t)np)) return rc;
+            if (int rc = dalloc(ctx, &ctx->op_ws, (size_t)wsb)) return rc;
+            if (int rc = dalloc(ctx, &ctx->op_u, nc * 16)) return rc;
+            if (int rc = dalloc(ctx, &ctx->op_out, nc * 16)) return rc;
+            if (int rc = dalloc(ctx, &ctx->op_psi, nc)) return rc;
+            if (int rc = dalloc(ctx, &ctx->Eb, nc)) return rc;
+            if (int rc = dalloc(ctx, &ctx->Bb, nc)) return rc;
+            if (int rc = dalloc(ctx, &ctx->Eo, nc)) return rc;
+            if (int rc = dalloc(ctx, &ctx->Bo, nc)) return rc;
+        }
+        for (long long i = 0; i < np; ++i)
+            if (!std::isfinite(op->params[i])) return fail(ctx, IPIC_E_ARG, "non-finite operator parameter");
+        HIPCK(ctx, hipMemcpyAsync(ctx->op_params, op->params, (size_t)np * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+        if (int rc = ipic_nop_prepare(ctx->op_params, ctx->op_wf, nullptr, ctx->stream)) return nop_fail(ctx, rc);
+        HIPCK(ctx, hipStreamSynchronize(ctx->stream));
+        ctx->op = *op;
+        ctx->op.params = nullptr;
+        ctx->op_scales = ipic_nop_scales{op->in_E, op->in_B, op->in_J, op->out_E, op->out_P};
+        ctx->op_count = 0;
+        ctx->op_on = true;
+        return 0;
+    });
+}
+
+int ipic_gpu_field_state(ipic_ctx* ctx, ipic_field_state* out) {
+    return guarded(ctx, [&]() -> int {
+        if (int rc = check_ctx(ctx)) return rc;
+        if (!out) return fail(ctx, IPIC_E_ARG, "null output");
+        if (!ctx->j_valid) {
+            if (int rc = launch_particles(ctx, MODE_DEPOSIT)) return rc;
+            ctx->j_valid = true;
+        }
+        if (int rc = launch_jsum(ctx)) return rc;
+        HIPCK(ctx, hipStreamSynchronize(ctx->stream));
+        out->E = (float*)ctx->E;
+        out->Bc = (float*)ctx->Bc;
+        out->J = (float*)ctx->J;
+        for (int k = 0; k < 3; ++k) {
+            out->E0[k] = ctx->g.E0[k];
+            out->B0[k] = ctx->g.B0[k];
+        }
+        out->uniform_background = (ctx->Ebg || ctx->Bcbg || ctx->SE || ctx->SB) ? 0 : 1;
+        return 0;
+    });
+}
+
+int ipic_gpu_num_particles(ipic_ctx* ctx, int32_t species, int64_t* n) {
+    return guarded(ctx, [&]() -> int {
+        if (int rc = check_ctx(ctx)) return rc;
+        if (!n) return fail(ctx, IPIC_E_ARG, "null output");
+        if (species >= ctx->d.nspecies) return fail(ctx, IPIC_E_ARG, "species out of range");
+        if (int rc = canonicalize(ctx)) return rc;
+        long long tot = 0;
+        for (int s = 0; s < ctx->d.nspecies; ++s) {
+            if (species >= 0 && s != species) continue;
+            long long t = 0;
+            if (int rc = read_total(ctx, s, &t)) return rc;
+            tot += t;
+        }
+        *n = tot;
+        return 0;
+    });
+}
+
+int ipic_gpu_cell_counts(ipic_ctx* ctx, int32_t species, int64_t* counts) {
+    return guarded(ctx, [&]() -> int {
+        if (int rc = check_ctx(ctx)) return rc;
+        if (!counts) return fail(ctx, IPIC_E_ARG, "null output");
+        if (species >= ctx->d.nspecies) return fail(ctx, IPIC_E_ARG, "species out of range");
+        if (int rc = canonicalize(ctx)) return rc;
+        const Geo& g = ctx->g;
+        const int nc = g.ncell;
+        std::vector<int> st(nc + 1);
+        std::memset(counts, 0, (size_t)nc * sizeof(int64_t));
+        for (int s = 0; s < ctx->d.nspecies; ++s) {
+            if (species >= 0 && s != species) continue;
+            HIPCK(ctx, hipMemcpy(st.data(), ctx->sp[s].start, (nc + 1) * sizeof(int), hipMemcpyDeviceToHost));
+            for (int cid = 0; cid < nc; ++cid) {
+                int x, y, z;
+                cid_xyz(g, cid, x, y, z);
+                counts[(x * g.n[1] + y) * g.n[2] + z] += st[cid + 1] - st[cid];
+            }
+        }
+        return 0;
+    });
+}
+
+int ipic_gpu_download_particles(ipic_ctx* ctx, int32_t species, double* pos, double* vel, int64_t* cell,
+                                uint32_t* ids) {
+    return guarded(ctx, [&]() -> int {
+        if (int rc = check_ctx(ctx)) return rc;
+        if (species < 0 || species >= ctx->d.nspecies) return fail(ctx, IPIC_E_ARG, "species out of range");
+        if (ids && !ctx->track) return fail(ctx, IPIC_E_STATE, "ids need IPIC_FLAG_TRACK_IDS");
+        if (int rc = canonicalize(ctx)) return rc;
+        const Geo& g = ctx->g;
+        const int nc = g.ncell;
+        Species& S = ctx->sp[species];
+        std::vector<int> st(nc + 1);
+        HIPCK(ctx, hipMemcpy(st.data(), S.start, (nc + 1) * sizeof(int), hipMemcpyDeviceToHost));
+        const long long n = st[nc];
+        std::vector<float> X((size_t)6 * std::max<long long>(n, 1));
+        for (int k = 0; k < 6 && n > 0; ++k)
+            HIPCK(ctx, hipMemcpy(&X[(size_t)k * n], S.X[S.cur] + (size_t)k * S.cap, (size_t)n * sizeof(float),
+                                 hipMemcpyDeviceToHost));
+        std::vector<uint32_t> hid(ids ? std::max<long long>(n, 1) : 0);
+        if (ids && n > 0) HIPCK(ctx, hipMemcpy(hid.data(), S.ids[S.cur], (size_t)n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        long long o = 0;
+        for (int x = 0; x < g.n[0]; ++x)
+            for (int y = 0; y < g.n[1]; ++y)
+                for (int z = 0; z < g.n[2]; ++z) {
+                    const int cid = host_cid(g, x, y, z);
+                    const int cc[3] = {x, y, z};
+                    for (int p = st[cid]; p < st[cid + 1]; ++p, ++o) {
+                        for (int k = 0; k < 3; ++k) {
+                            if (pos) pos[3 * o + k] = ctx->d.origin[k] + ((double)cc[k] + (double)X[(size_t)k * n + p]) * ctx->d.cell_width[k];
+                            if (vel) vel[3 * o + k] = (double)X[(size_t)(3 + k) * n + p];
+                        }
+                        if (cell) cell[o] = ((long long)x * g.n[1] + y) * g.n[2] + z;
+                        if (ids) ids[o] = hid[p];
+                    }
+                }
+        return 0;
+    });
+}
+
+int ipic_gpu_download_fields(ipic_ctx* ctx, double* E, double* Bc, double* Bn, double* J) {
+    return guarded(ctx, [&]() -> int {
+        if (int rc = check_ctx(ctx)) return rc;
+        const Geo& g = ctx->g;
+        const int nc = g.ncell;
+        std::vector<double> tB;
+        if (Bn && !Bc) tB.resize((size_t)nc * 3);
+        double* bc = Bc ? Bc : (Bn ? tB.data() : nullptr);
+        if (int rc = total_fields(ctx, E, bc)) return rc;
+        if (Bn) {
+            // interpC2N (field.h:234-247) of the total Bc, FP64
+            const int* n = g.n;
+            for (int x = 0; x < n[0]; ++x)
+                for (int y = 0; y < n[1]; ++y)
+                    for (int z = 0; z < n[2]; ++z) {
+                        const size_t i = (size_t)((x * n[1] + y) * n[2] + z);
+                        for (int k = 0; k < 3; ++k) {
+                            double acc = 0;
+                            for (int a = 0; a < 2; ++a)
+                                for (int b = 0; b < 2; ++b)
+                                    for (int c = 0; c < 2; ++c) {
+                                        const int xx = (x - a + n[0]) % n[0], yy = (y - b + n[1]) % n[1],
+                                                  zz = (z - c + n[2]) % n[2];
+                                        acc += bc[3 * ((xx * n[1] + yy) * n[2] + zz) + k];
+                                    }
+                            Bn[3 * i + k] = 0.125 * acc;
+                        }
+                    }
+        }
+        if (J) {
+            std::vector<float4> h(nc);
+            HIPCK(ctx, hipMemcpy(h.data(), ctx->J, nc * sizeof(float4), hipMemcpyDeviceToHost));
+            for (int i = 0; i < nc; ++i) {
+                J[3 * i] = h[i].x;
+                J[3 * i + 1] = h[i].y;
+                J[3 * i + 2] = h[i].z;
+            }
+        }
+        return 0;
+    });
 }
 
 int ipic_gpu_conservation(ipic_ctx* ctx, ipic_conservation* out) {
-    if (int rc = check_ctx(ctx)) return rc;
-    if (!out) return fail(ctx, IPIC_E_ARG, "null output");
-    ipic_conservation r{};
-    std::vector<double> h((size_t)ctx->red_blocks * 5);
-    for (int s = 0; s < ctx->d.nspecies; ++s) {
-        Species& S = ctx->sp[s];
-        long long nslots = 0;
-        if (int rc = read_slots(ctx, s, &nslots)) return rc;
-        if (nslots == 0) continue;
-        const double mass = S.q / S.qom;
-        hipLaunchKernelGGL(k_reduce_particles, dim3(ctx->red_blocks), dim3(256), 0, ctx->stream, S.X[S.cur], S.cap,
-                           nslots, mass, ctx->red);
-        HIPCK(ctx, hipGetLastError());
-        HIPCK(ctx, hipMemcpyAsync(h.data(), ctx->red, (size_t)ctx->red_blocks * 3 * sizeof(double),
-                                  hipMemcpyDeviceToHost, ctx->stream));
-        HIPCK(ctx, hipStreamSynchronize(ctx->stream));
-        double ke = 0, mo = 0, cnt = 0;
-        for (int b = 0; b < ctx->red_blocks; ++b) {
-            ke += h[3 * b];
-            mo += h[3 * b + 1];
-            cnt += h[3 * b + 2];
+    return guarded(ctx, [&]() -> int {
+        if (int rc = check_ctx(ctx)) return rc;
+        if (!out) return fail(ctx, IPIC_E_ARG, "null output");
+        ipic_conservation r{};
+        std::vector<double> h((size_t)ctx->red_blocks * 5);
+        for (int s = 0; s < ctx->d.nspecies; ++s) {
+            Species& S = ctx->sp[s];
+            long long nslots = 0;
+            if (int rc = read_slots(ctx, s, &nslots)) return rc;
+            if (nslots == 0) continue;
+            const double mass = S.q / S.qom;
+            hipLaunchKernelGGL(k_reduce_particles, dim3(ctx->red_blocks), dim3(256), 0, ctx->stream, S.X[S.cur], S.cap,
+                               nslots, mass, ctx->red);
+            HIPCK(ctx, hipGetLastError());
+            HIPCK(ctx, hipMemcpyAsync(h.data(), ctx->red, (size_t)ctx->red_blocks * 3 * sizeof(double),
+                                      hipMemcpyDeviceToHost, ctx->stream));
+            HIPCK(ctx, hipStreamSynchronize(ctx->stream));
+            double ke = 0, mo = 0, cnt = 0;
+            for (int b = 0; b < ctx->red_blocks; ++b) {
+                ke += h[3 * b];
+                mo += h[3 * b + 1];
+                cnt += h[3 * b + 2];
+            }
+            r.kinetic_energy += ke;
+            r.momentum += mo;
+            r.charge += cnt * S.q;
+            r.particles += (int64_t)cnt;
         }
-        r.kinetic_energy += ke;
-        r.momentum += mo;
-        r.charge += cnt * S.q;
-        r.particles += (int64_t)cnt;
-    }
-    FieldArgs F = field_args(ctx);
-    hipLaunchKernelGGL(k_reduce_fields, dim3(ctx->red_blocks), dim3(256), 0, ctx->stream, F, ctx->red);
-    HIPCK(ctx, hipGetLastError());
-    HIPCK(ctx, hipMemcpyAsync(h.data(), ctx->red, (size_t)ctx->red_blocks * 5 * sizeof(double), hipMemcpyDeviceToHost,
-                              ctx->stream));
-    HIPCK(ctx, hipStreamSynchronize(ctx->stream));
-    double se = 0, sb = 0, sej = 0, sd2 = 0, md = 0;
-    for (int b = 0; b < ctx->red_blocks; ++b) {
-        se += h[5 * b];
-        sb += h[5 * b + 1];
-        sej += h[5 * b + 2];
-        sd2 += h[5 * b + 3];
-        md = std::max(md, h[5 * b + 4]);
-    }
-    const double vol = ctx->d.cell_width[0] * ctx->d.cell_width[1] * ctx->d.cell_width[2];
-    r.field_energy_E = 0.5 * se * vol;
-    r.field_energy_B = 0.5 * ctx->d.c * ctx->d.c * sb * vol;
-    r.EdotJ = sej * vol;
-    r.divB_max = md;
-    r.divB_l2 = std::sqrt(sd2);
-    *out = r;
-    return 0;
+        FieldArgs F = field_args(ctx);
+        hipLaunchKernelGGL(k_reduce_fields, dim3(ctx->red_blocks), dim3(256), 0, ctx->stream, F, ctx->red);
+        HIPCK(ctx, hipGetLastError());
+        HIPCK(ctx, hipMemcpyAsync(h.data(), ctx->red, (size_t)ctx->red_blocks * 5 * sizeof(double), hipMemcpyDeviceToHost,
+                                  ctx->stream));
+        HIPCK(ctx, hipStreamSynchronize(ctx->stream));
+        double se = 0, sb = 0, sej = 0, sd2 = 0, md = 0;
+        for (int b = 0; b < ctx->red_blocks; ++b) {
+            se += h[5 * b];
+            sb += h[5 * b + 1];
+            sej += h[5 * b + 2];
+            sd2 += h[5 * b + 3];
+            md = std::max(md, h[5 * b + 4]);
+        }
+        const double vol = ctx->d.cell_width[0] * ctx->d.cell_width[1] * ctx->d.cell_width[2];
+        r.field_energy_E = 0.5 * se * vol;
+        r.field_energy_B = 0.5 * ctx->d.c * ctx->d.c * sb * vol;
+        r.EdotJ = sej * vol;
+        r.divB_max = md;
+        r.divB_l2 = std::sqrt(sd2);
+        *out = r;
+        return 0;
+    });
 }
 
 }  // extern "C"

@@ -28,7 +28,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from . import FLAG_REFLECT, Simulation, ref_distribution
+from . import FLAG_REFLECT, Simulation, desc_for, field_init, field_init_host, ref_distribution
 from . import io as ipic_io
 
 E_CHARGE = 1.602176565e-19   # elementary charge, C (ipic3d.cxx:70)
@@ -132,29 +132,18 @@ def driver_case(params: Parameters, base_name: str = "") -> DriverCase:
                       ncycles=params.ncycles, field_output_cycle=params.field_output_cycle)
 
 
+def case_field_init(case: DriverCase):
+    """InitProperties / UniverseProperties of the driver's field (ipic3d.cxx:41-85): B0 and the
+    drift of the configuration, the planet at the object centre, external B_z."""
+    return field_init(drift=case.drift, B0=case.B0, object_center=case.object_center,
+                      planet_radius=case.planet_radius, external_Bz=case.external_B[2])
+
+
 def initial_field_energies(case: DriverCase):
-    """getFieldEnergy of E and of B + Bext (simulator.h:75-79) over the nodes initFields fills,
-    locations origin + i * width for i in [0, N]^3 (field.h:80-110, universe_properties.h:132-137),
-    with the dipole field of getDipoleFieldAt (field.h:43-75)."""
-    axes = [case.origin[k] + np.arange(case.n[k] + 1, dtype=np.float64) * case.cell_width[k] for k in range(3)]
-    X, Y, Z = np.meshgrid(*axes, indexing="ij")
-    d = [X - case.object_center[0], Y - case.object_center[1], Z - case.object_center[2]]
-    r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2]
-    a = case.planet_radius
-    outside = r2 > a * a
-    with np.errstate(divide="ignore", invalid="ignore"):
-        fac = np.where(outside, -case.external_B[2] * math.pow(a, 3) / np.power(r2, 2.5), 0.0)
-    Bext = [3.0 * d[0] * d[2] * fac, 3.0 * d[1] * d[2] * fac, (2.0 * d[2] * d[2] - d[0] * d[0] - d[1] * d[1]) * fac]
-    u, B0 = case.drift, case.B0
-    E = [-(u[1] * B0[2] - u[2] * B0[1]), -(u[2] * B0[0] - u[0] * B0[2]), -(u[0] * B0[1] - u[1] * B0[0])]
-    B = [B0[k] + Bext[k] for k in range(3)]
-    vol = 0.5 * case.cell_width[0] * case.cell_width[1] * case.cell_width[2]
-    four_pi = 16.0 * math.atan(1.0)
-    nodes = X.size
-    e_energy = float(vol * (E[0] ** 2 + E[1] ** 2 + E[2] ** 2) / four_pi * nodes)
-    BB = [B[k] + Bext[k] for k in range(3)]
-    b_energy = float(np.sum(vol * (BB[0] ** 2 + BB[1] ** 2 + BB[2] ** 2) / four_pi))
-    return e_energy, b_energy
+    """getFieldEnergy of E and of B + Bext (simulator.h:75-79) over the nodes initFields fills
+    (field.h:81-118), from the library's host initialisation (ipic_field_init_host)."""
+    desc = desc_for(case.n, case.cell_width, case.origin)
+    return field_init_host(desc, case_field_init(case))["energies"]
 
 
 def make_simulation(case: DriverCase, tile=(0, 0, 0), flags=0) -> Simulation:

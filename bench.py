@@ -40,18 +40,59 @@ def algorithmic_bytes_per_particle(ppc: float) -> float:
     return 48.0 + 36.0 / ppc
 
 
+PUSH_SOURCE = os.path.join(ROOT, "allscale_ipic3d_amd", "csrc", "ipic_gpu.hip")
+
+
+def source_sha256() -> str:
+    import hashlib
+    with open(PUSH_SOURCE, "rb") as f:
+        return hashlib.sha256(f.read()).hexdigest()
+
+
 def load_traffic(cells: int, ppc: int):
-    """HBM bytes per push launch measured by rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE,
-    scaled by the copy-kernel calibration), recorded in profiles/ for this workload."""
+    """HBM bytes per push launch measured by rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE, scaled by
+    the copy-kernel calibration), recorded in profiles/ for this workload.  The record carries
+    the hash of the kernel source it was measured on; a record of another source is stale and
+    gives no figure.  Returns (bytes or None, status)."""
     path = os.path.join(ROOT, "profiles", f"push_kernel_{cells}c_{ppc}ppc_pmc.json")
     if not os.path.exists(path):
-        return None
+        return None, "no record"
     try:
         with open(path) as f:
             d = json.load(f)
-        return float(d["hbm_bytes_per_launch"])
-    except Exception:
-        return None
+    except (OSError, ValueError):
+        return None, "unreadable record"
+    if d.get("source_sha256") != source_sha256():
+        return None, "stale: measured on another version of csrc/ipic_gpu.hip"
+    return float(d["hbm_bytes_per_launch"]), "measured on this source"
+
+
+def pmc_summary(fetch_csv: str, write_csv: str, fetch_scale: float, write_scale: float, out: str, command: str):
+    """Summarises two rocprofv3 --pmc counter_collection.csv files (FETCH_SIZE, WRITE_SIZE, KiB per
+    dispatch) of the push kernel into the record load_traffic reads, stamped with the source hash."""
+    import csv
+
+    def per_dispatch(path, counter):
+        vals = {}
+        with open(path) as f:
+            for row in csv.DictReader(f):
+                if "k_particles<0" in row["Kernel_Name"] and row["Counter_Name"] == counter:
+                    key = row.get("Dispatch_Id") or row.get("Correlation_Id")
+                    vals[key] = vals.get(key, 0.0) + float(row["Counter_Value"])
+        v = list(vals.values())
+        return {"dispatches": len(v), "avg_kib": sum(v) / len(v), "min_kib": min(v), "max_kib": max(v)}
+
+    fe = per_dispatch(fetch_csv, "FETCH_SIZE")
+    wr = per_dispatch(write_csv, "WRITE_SIZE")
+    rd, wb = fe["avg_kib"] * 1024 * fetch_scale, wr["avg_kib"] * 1024 * write_scale
+    rec = {"command": command, "kernel": "k_particles<0 (push)", "source_sha256": source_sha256(),
+           "FETCH_SIZE_kib_per_dispatch": fe, "WRITE_SIZE_kib_per_dispatch": wr,
+           "fetch_scale": fetch_scale, "write_scale": write_scale,
+           "note": "scales from a 1 GiB device copy (python bench.py --calibrate-copy under the same counters)",
+           "hbm_read_bytes_per_launch": rd, "hbm_write_bytes_per_launch": wb, "hbm_bytes_per_launch": rd + wb}
+    with open(out, "w") as f:
+        json.dump(rec, f, indent=1)
+    print(json.dumps(rec))
 
 
 def make_sim(G, cells, ppc, seed, flags=0, tile=(0, 0, 0)):
@@ -68,40 +109,101 @@ def make_sim(G, cells, ppc, seed, flags=0, tile=(0, 0, 0)):
     return sim
 
 
-def cpu_baseline(cells: int, ppc: int, budget_s: float = 12.0):
-    """The FP64 oracle (a CPU restatement of the reference step) on this host's cores, on a
-    bounded sample of the same workload: a 64^3 box (1/8 of the cells) with the same
-    particles per cell and physics, timed over as many steps as fit in the budget."""
+def host_cpus():
+    """CPUs this process may use (its affinity mask, capped by OMP_NUM_THREADS when set, which the
+    GPU box sets to the job's CPU share), the machine's total and its model."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:  # pragma: no cover
+        avail = os.cpu_count() or 1
+    omp = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:  # pragma: no cover
+        pass
+    return {"threads": min(avail, omp) if omp > 0 else avail, "affinity_cpus": avail,
+            "machine_cpus": os.cpu_count(), "omp_num_threads": omp or None, "model": model}
+
+
+def cpu_leg(G, cells: int, ppc: int, seed: int, fp64_steps: int = 3, budget_s: float = 10.0):
+    """The FP64 oracle (a CPU restatement of the reference step) on this host's cores.
+
+    Full size: the oracle takes exactly the particles and fields of a GPU run of the benchmark
+    workload (same seed; FP32 values widened to FP64); both advance `fp64_steps` steps.  The
+    oracle's steps are timed on all the job's cores (the CPU baseline at the workload's size) and
+    the two states are compared (fields, per-cell particle sums, conservation sums: the
+    FP32-vs-FP64 error of the measured workload).  A 64^3 sample of the same physics is also timed
+    over more steps, and on one core."""
     import oracle as O
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
-    n = min(cells, 64)
+    import oracle.compare as OC
+    cpu = host_cpus()
+    threads = cpu["threads"]
     half = ppc // 2
     q = 1.0 / (half * FOUR_PI)
-    u = O.Universe(n, dt=0.05)
-    u.init_species(0, 1, half, -q, -1.0, (0.05,) * 3)
-    u.init_species(1, 1, ppc - half, q, 1.0 / 64, (0.006,) * 3)
-    rng = np.random.default_rng(1)
-    u.set_fields(rng.normal(0.0, 1e-3, size=(n ** 3, 3)), np.tile([0.0, 0.2, 1.0], (n ** 3, 1)))
+    species = ((-q, -1.0), (q, 1.0 / 64))
+    sim = make_sim(G, cells, ppc, seed)
+    rng = np.random.default_rng(seed)
+    n3 = cells ** 3
+    E = rng.normal(0.0, 1e-3, size=(n3, 3))
+    Bc = np.tile([0.0, 0.2, 1.0], (n3, 1))
+    t0 = time.perf_counter()
+    u = OC.oracle_from_gpu(sim, species, E, Bc, dt=0.05, c=1.0, field_substeps=1)
+    log(f"oracle holds the GPU state ({u.num_particles()} particles) after {time.perf_counter() - t0:.1f} s")
     npart = u.num_particles()
+    t0 = time.perf_counter()
+    u.step(fp64_steps, threads)
+    el_full = time.perf_counter() - t0
+    sim.step(fp64_steps)
+    res = OC.compare(u, sim, species)
+    u.close()
+    sim.close()
+    worst = max([*res["fields"].values(), *[p[k] for p in res["particles"] for k in ("offset_max", "velocity_max")],
+                 *[v for k, v in res["conservation"].items() if k not in ("particles", "charge")]])
+    fp64 = {"workload": "the benchmark workload (same seed and inputs)", "steps": fp64_steps,
+            "tolerance": 1e-5, "max_rel_error": worst, **res,
+            "definition": "fields: relative l2 error of E, Bc, Bn, J; particles per species: offset_max = max over "
+                          "cells of |sum of offsets GPU - FP64| / count (cells), velocity_max = max over cells of "
+                          "|sum of v GPU - FP64| / sum of |v| (cells whose counts agree; flip_* = cells/particles "
+                          "on the other side of a face); conservation: relative error of the FP64 sums"}
+    log("fp64 error:", json.dumps(fp64))
+
+    # 64^3 sample of the same physics: more steps, and one core
+    n = min(cells, 64)
+    us = O.Universe(n, dt=0.05)
+    us.init_species(0, 1, half, -q, -1.0, (0.05,) * 3)
+    us.init_species(1, 1, ppc - half, q, 1.0 / 64, (0.006,) * 3)
+    rng = np.random.default_rng(1)
+    us.set_fields(rng.normal(0.0, 1e-3, size=(n ** 3, 3)), np.tile([0.0, 0.2, 1.0], (n ** 3, 1)))
+    ns = us.num_particles()
 
     def timed(nthreads, budget, max_steps):
         steps, t0 = 0, time.perf_counter()
         while True:
-            u.step(1, nthreads)
+            us.step(1, nthreads)
             steps += 1
             el = time.perf_counter() - t0
             if el > budget or steps >= max_steps:
                 return steps, el
 
-    u.step(1, threads)  # warm-up
+    us.step(1, threads)  # warm-up
     steps, el = timed(threads, budget_s, 40)
-    steps1, el1 = timed(1, budget_s / 2, 5)  # the same step on one core
-    u.close()
-    return {"value": npart * steps / el, "unit": "particle updates/s", "cores": threads, "kind": "port",
-            "value_1core": npart * steps1 / el1,
-            "sample": f"FP64 oracle step (deposit, leapfrog, gather+Boris, rebin), {n}^3 cells x {ppc} ppc "
-                      f"({npart} particles), {steps} steps in {el:.1f} s on {threads} threads; "
-                      f"{steps1} steps in {el1:.1f} s on 1 thread"}
+    steps1, el1 = timed(1, budget_s / 2, 5)
+    us.close()
+    base = {"value": npart * fp64_steps / el_full, "unit": "particle updates/s", "cores": threads, "kind": "port",
+            "sample": f"FP64 oracle step (deposit, leapfrog, gather+Boris, rebin) on the full workload, "
+                      f"{cells}^3 cells x {ppc} ppc ({npart} particles), {fp64_steps} steps in {el_full:.1f} s on "
+                      f"{threads} threads",
+            "host": cpu,
+            "sample_64c": {"value": ns * steps / el, "cores": threads,
+                           "sample": f"{n}^3 cells x {ppc} ppc ({ns} particles), {steps} steps in {el:.1f} s"},
+            "value_1core": ns * steps1 / el1,
+            "sample_1core": f"{n}^3 cells x {ppc} ppc, {steps1} steps in {el1:.1f} s on 1 thread"}
+    return base, fp64
 
 
 def copy_peak_gbs(G):
@@ -196,18 +298,12 @@ def other_configs(G):
     c = n / 2.0
     x = np.arange(n) + 0.5
     X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
-    d = np.stack([X - c, Y - c, Z - c], axis=-1).reshape(-1, 3)
-    r2 = np.sum(d * d, axis=1)
-    with np.errstate(divide="ignore", invalid="ignore"):
-        fac = 1.0 / r2 ** 2.5
-    B = np.stack([3 * d[:, 0] * d[:, 2] * fac, 3 * d[:, 1] * d[:, 2] * fac,
-                  (2 * d[:, 2] ** 2 - d[:, 0] ** 2 - d[:, 1] ** 2) * fac], axis=1)
-    B[r2 <= 16.0] = 0.0
+    r2 = ((X - c) ** 2 + (Y - c) ** 2 + (Z - c) ** 2).reshape(-1)
     ppc = np.rint(30 * (1 + 9 * np.exp(-r2 / 400.0))).astype(np.int32)
     q = -1.0 / (30 * FOUR_PI)
     sim = G.Simulation(n, dt=0.1, species=((q, -1.0),), sink_center=(c, c, c), sink_radius=4.0)
     sim.init_particles(0, ppc, (0.05,) * 3, drift=(0.1, 0.0, 0.0), seed=3)
-    sim.upload_fields(np.zeros((n ** 3, 3)), B)
+    sim.init_fields(G.dipole_field_init(1.0, (c, c, c), 4.0))
     sim.step(2)
     st = sim.step(5)
     t = (st["first_step_s"] + st["remaining_steps_s"]) / 5
@@ -249,7 +345,16 @@ def main():
     ap.add_argument("--no-other-configs", action="store_true", help="skip the per-step times of the other configurations")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy 10 times (rocprof counter calibration)")
+    ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="summarise two rocprofv3 --pmc counter files of the push kernel into the traffic record")
+    ap.add_argument("--pmc-scales", nargs=2, type=float, default=(2.0, 1.0), metavar=("FETCH", "WRITE"))
+    ap.add_argument("--pmc-command", default="", help="the profiling command, recorded with the summary")
     args = ap.parse_args()
+    if args.pmc_summary:
+        pmc_summary(args.pmc_summary[0], args.pmc_summary[1], args.pmc_scales[0], args.pmc_scales[1],
+                    os.path.join(ROOT, "profiles", f"push_kernel_{args.cells}c_{args.ppc}ppc_pmc.json"),
+                    args.pmc_command)
+        return
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -311,10 +416,11 @@ def main():
     if not args.no_other_configs:
         others = other_configs(G)
         log("other configurations:", others)
-    cpu = None if args.no_cpu_baseline else cpu_baseline(cells, ppc)
+    cpu, fp64 = (None, None) if args.no_cpu_baseline else cpu_leg(G, cells, ppc, args.seed)
     if cpu:
         log("cpu baseline:", cpu)
 
+    traffic, traffic_status = load_traffic(cells, ppc)
     out = {
         "metric": METRIC,
         "value": value,
@@ -346,7 +452,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": load_traffic(cells, ppc),
+            "traffic": traffic,
+            "traffic_status": traffic_status,
             "algorithmic_bytes_per_particle": Bp,
             "algorithmic_bytes_per_launch": alg_bytes,
             "avg_launch_ms": push_ms,
@@ -354,6 +461,7 @@ def main():
             "step_breakdown_ms": {"field": field_ms, "push": push_ms, "scan": scan_ms},
         },
         "cpu_baseline": cpu,
+        "fp64_error": fp64,
         "surrogate": sur,
         "other_configs": others,
     }

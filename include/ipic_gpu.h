@@ -245,6 +245,35 @@ const char* ipic_ref_last_error(void);
  * receives the phase-1 counts. */
 int ipic_gpu_init_reference(ipic_ctx* ctx, int32_t species, const ipic_ref_distribution* dist, int64_t* counts);
 
+/* The initial field of the reference's dipole use case (UseCase::Dipole):
+ *   getDipoleFieldAt (field.h:43-78): E = -drift x B0; B = B0 + Bext with
+ *     Bext = -external_Bz a^3 (3 x z, 3 y z, 2 z^2 - x^2 - y^2) / r^5 for r > a (r = location -
+ *     object_center, a = planet_radius), Bext = 0 inside the planet;
+ *   initFields (field.h:81-118): that field at the node locations origin + i * cell_width;
+ *   initBcFields (field.h:120-138): Bc of cell i = 1/8 of node B over nodes i + {0,1}^3
+ *     (interpN2C, field.h:252-265).
+ * The members are the InitProperties / UniverseProperties fields these functions read. */
+typedef struct ipic_field_init {
+    double drift[3];              /* InitProperties::driftVelocity[0] */
+    double B0[3];                 /* InitProperties::magneticField */
+    double object_center[3];      /* UniverseProperties::objectCenter */
+    double planet_radius;         /* UniverseProperties::planetRadius */
+    double external_Bz;           /* UniverseProperties::externalMagneticField.z */
+} ipic_field_init;
+
+/* Host computation on desc's grid: E and node B at the N^3 nodes, Bc at the N^3 centres
+ * (n^3 x 3 doubles each, row-major, any may be NULL; the last centre layer averages the node
+ * locations at N as the reference's does).  energies (nullable) receives getFieldEnergy
+ * (field.h:443-459) of E and of B + Bext over the (N+1)^3 nodes of initFields, the values
+ * writeOutputData logs (simulator.h:73-79). */
+int ipic_field_init_host(const ipic_desc* desc, const ipic_field_init* init, double* E, double* Bn, double* Bc,
+                         double energies[2]);
+/* Message of the last failed ipic_field_init_host call on this thread. */
+const char* ipic_field_init_last_error(void);
+/* initFields + initBcFields as the context's field state: ipic_gpu_upload_fields of the
+ * host computation's E and Bc.  energies as above (nullable). */
+int ipic_gpu_init_fields(ipic_ctx* ctx, const ipic_field_init* init, double energies[2]);
+
 /* Device properties used by the tests and benchmark. */
 int ipic_gpu_device_info(char* name, int32_t name_len, int32_t* cu_count, int64_t* hbm_bytes);
 

@@ -90,6 +90,11 @@ def test_refused_calls_leave_the_context_usable():
     big = G.Simulation((128, 128, 128), dt=0.1, species=((-0.01, -1.0),))
     with pytest.raises(G.IpicError, match="178956970"):
         big.init_particles(0, 86, (0.1, 0.1, 0.1))
+    # the same limit on upload, checked before the host builds any buffer (the arrays passed
+    # are never read: the call is refused first)
+    tiny = np.zeros((1, 3))
+    rc = big._L.ipic_gpu_upload_particles(big._h, 0, 178956971, G._ptr(tiny), G._ptr(tiny))
+    assert rc == -4 and "178956970" in big._L.ipic_gpu_last_error(big._h).decode()
     big.init_particles(0, 2, (0.1, 0.1, 0.1))
     assert big.step(1)["particles"] == 2 * 128 ** 3
     big.close()

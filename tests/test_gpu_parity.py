@@ -69,10 +69,13 @@ def test_config1_one_step():
     _counts_close(u, sim, 2)
 
 
-def test_config1_twenty_steps():
+@pytest.mark.parametrize("track_ids", [True, False])
+def test_config1_twenty_steps(track_ids):
+    """Without ids this is the particle-kernel instance the benchmark times (8x8x8 tiles, no
+    sink, reflection or dipole options)."""
     case = make_case(n=8, ppc=64, species=((-1.0, 0.02),), dt=0.1, B=(0, 0, 1.0), seed=0)
     u = case[0]
-    sim = gpu_sim_for(case)
+    sim = gpu_sim_for(case, track_ids=track_ids)
     u.step(20)
     st = sim.step(20)
     assert st["steps"] == 20 and st["particles"] == 8 ** 3 * 64
@@ -89,13 +92,14 @@ def test_config1_twenty_steps():
     assert cu["field_energy_B"] == pytest.approx(wB, rel=1e-5)
 
 
-def test_two_species_config2_shape_small():
+@pytest.mark.parametrize("track_ids", [True, False])
+def test_two_species_config2_shape_small(track_ids):
     # config-2 physics on a small box: electrons q/m=-1 (vth 0.05) and ions q/m=1/64
     # (vth 0.006), B=(0,0.2,1), E ~ N(0, 1e-3^2), dt=0.05
     case = make_case(n=(8, 8, 8), ppc=16, species=((-1.0, 0.05), (1.0 / 64, 0.006)), dt=0.05,
                      B=(0, 0.2, 1.0), E_sigma=1e-3, seed=3)
     u = case[0]
-    sim = gpu_sim_for(case)
+    sim = gpu_sim_for(case, track_ids=track_ids)
     u.step(10)
     sim.step(10)
     _compare_fields(u, sim)
@@ -116,8 +120,9 @@ def test_two_species_config2_shape_uniform_background_mode():
     _compare_particles(u, sim, [-1.0, 1.0 / 64], (8, 8, 16), (1, 1, 1), 0.05)
 
 
+@pytest.mark.parametrize("track_ids", [True, False])
 @pytest.mark.parametrize("tile", [(0, 0, 0), (2, 2, 2)])
-def test_tile_slices_and_single_block_tiles(tile):
+def test_tile_slices_and_single_block_tiles(tile, track_ids):
     """16^3 cells x 16 particles per cell of each of two species, 6 steps.  With the default
     8^3 tiles (8 tiles) the particle kernel splits every tile's cells among many blocks, whose
     J partials the E pass sums; with 2^3 tiles (512 tiles) each tile is one block.  Both match
@@ -125,13 +130,60 @@ def test_tile_slices_and_single_block_tiles(tile):
     case = make_case(n=(16, 16, 16), ppc=16, species=((-1.0, 0.05), (1.0 / 64, 0.006)), dt=0.05,
                      B=(0, 0.2, 1.0), E_sigma=1e-3, seed=9)
     u = case[0]
-    sim = gpu_sim_for(case, tile=tile)
+    sim = gpu_sim_for(case, tile=tile, track_ids=track_ids)
     u.step(6)
     st = sim.step(6)
     assert st["particles"] == 2 * 16 ** 3 * 16  # 16 per cell of each species
     _compare_fields(u, sim)
     _compare_particles(u, sim, [-1.0, 1.0 / 64], (16, 16, 16), (1, 1, 1), 0.05)
     _counts_close(u, sim, 4)
+
+
+def test_benchmarked_instance_two_species_16c_32ppc():
+    """The benchmark's workload at 16^3 cells: two species, 16 + 16 particles per cell, config-2
+    physics, no ids, 8x8x8 tiles (the timed particle-kernel instance, here with tile slices),
+    10 steps, particle by particle against the oracle."""
+    case = make_case(n=(16, 16, 16), ppc=16, species=((-1.0, 0.05), (1.0 / 64, 0.006)), dt=0.05,
+                     B=(0, 0.2, 1.0), E_sigma=1e-3, seed=17)
+    u = case[0]
+    sim = gpu_sim_for(case, track_ids=False)
+    u.step(10)
+    st = sim.step(10)
+    assert st["particles"] == 2 * 16 * 16 ** 3
+    _compare_fields(u, sim)
+    _compare_particles(u, sim, [-1.0, 1.0 / 64], (16, 16, 16), (1, 1, 1), 0.05)
+    _counts_close(u, sim, 8)
+    cu = sim.conservation()
+    assert cu["kinetic_energy"] == pytest.approx(u.kinetic_energy(), rel=1e-5)
+    assert cu["momentum"] == pytest.approx(u.momentum(), rel=1e-5)
+
+
+def test_static_field_keeps_j_of_the_current_deposit():
+    """Static field (field_substeps = 0, field.h:272-284): J is still formed from each step's
+    deposit, so it matches the oracle after step(n) and after a particle step followed by a
+    field-state read; deposits beyond a tile (fast particles on a 3^3 grid, one tile) do not
+    accumulate across steps."""
+    rng = np.random.default_rng(13)
+    n = (3, 3, 3)
+    u = O.Universe(n, dt=0.5, c=1.0, field_substeps=0)
+    N = 30 * 27
+    pos = rng.uniform(size=(N, 3)) * 3.0
+    vel = rng.uniform(-2.5, 2.5, size=(N, 3)) * 0.3
+    u.add_particles(O.make_particles(pos, vel, 0.01, -1.0))
+    Bc = np.tile([0.0, 0.0, 0.3], (27, 1))
+    u.set_fields(np.zeros((27, 3)), Bc)
+    import allscale_ipic3d_amd as G
+    sim = G.Simulation(n, dt=0.5, species=((0.01, -1.0),), field_substeps=0)
+    sim.upload_particles(0, pos, vel)
+    sim.upload_fields(np.zeros((27, 3)), Bc)
+    u.step(3)
+    sim.step(3)
+    _compare_fields(u, sim)
+    # one more particle phase: the J of the new state, formed on demand
+    u.move(); u.rebin(); u.deposit()
+    sim.particle_step()
+    sim.field_state()
+    _compare_fields(u, sim, names=("J",))
 
 
 def test_fields_only_leapfrog():
@@ -339,35 +391,35 @@ def test_full_size_invariants_config2():
         assert off.min() >= 0.0 and off.max() <= 1.0
 
 
-def _dipole_centres(n, moment=1.0, radius=2.0):
-    """Bc of a z dipole at the domain centre evaluated at cell centres with the field.h:64-71
-    form, B1z * a^3 chosen so the moment is +moment (zero inside the radius)."""
-    c = (np.asarray(n, dtype=np.float64)) / 2.0
-    x, y, z = np.meshgrid(*[np.arange(k) + 0.5 for k in n], indexing="ij")
-    d = np.stack([x - c[0], y - c[1], z - c[2]], axis=-1).reshape(-1, 3)
-    r2 = np.sum(d * d, axis=1)
-    with np.errstate(divide="ignore", invalid="ignore"):
-        fac = moment / r2 ** 2.5
-    B = np.stack([3 * d[:, 0] * d[:, 2] * fac, 3 * d[:, 1] * d[:, 2] * fac,
-                  (2 * d[:, 2] ** 2 - d[:, 0] ** 2 - d[:, 1] ** 2) * fac], axis=1)
-    B[r2 <= radius * radius] = 0.0
-    return B, c
+def _dipole_init(n, moment=1.0, radius=2.0):
+    """A z dipole of the given moment at the domain centre (B = moment (3xz, 3yz, 2z^2-x^2-y^2) /
+    r^5 outside the radius, zero inside) through the library's initFields / initBcFields."""
+    import allscale_ipic3d_amd as G
+    c = np.asarray(n, dtype=np.float64) / 2.0
+    return G.dipole_field_init(moment, tuple(c), radius), c
 
 
 def test_config3_dipole_drift_uneven_load_small():
     """The uneven-load configuration at 32^3: a centred z dipole (moment 1) as the magnetic
     background, drift 0.1 along x, density 8 * (1 + 9 exp(-r^2 / 5^2)) particles per cell
     (8 to 80), a sink of radius 2 at the centre; 6 steps against the oracle."""
+    import allscale_ipic3d_amd as G
     n = (32, 32, 32)
-    B, c = _dipole_centres(n)
+    init, c = _dipole_init(n)
+    fi = G.field_init_host(G.desc_for(n), init)
     x, y, z = np.meshgrid(*[np.arange(k) + 0.5 for k in n], indexing="ij")
     r2 = ((x - c[0]) ** 2 + (y - c[1]) ** 2 + (z - c[2]) ** 2).reshape(-1)
     ppc = np.rint(8 * (1 + 9 * np.exp(-r2 / 25.0))).astype(np.int64)
     case = make_case(n=n, ppc=8, species=((-1.0, 0.05),), dt=0.1, B=(0.0, 0.0, 0.0), seed=9,
                      drift=(0.1, 0.0, 0.0), sink_center=tuple(c), sink_radius=2.0, ppc_per_cell=ppc)
-    u, sp, parts, E, Bc = case
-    u.set_fields(E, B)
-    sim = gpu_sim_for((u, sp, parts, E, B))
+    u, sp, parts, _, _ = case
+    u.set_fields(fi["E"], fi["Bc"])
+    sim = gpu_sim_for((u, sp, parts, fi["E"], fi["Bc"]))
+    # the same field through the GPU entry point: the state is the host computation's
+    assert sim.init_fields(init) == fi["energies"]
+    f0 = sim.download_fields()
+    np.testing.assert_array_equal(f0["Bc"], fi["Bc"])
+    np.testing.assert_array_equal(f0["E"], fi["E"])
     u.step(6)
     st = sim.step(6)
     print("particles", st["particles"], "removed", st["removed"], "max ppc", int(ppc.max()))
@@ -387,14 +439,14 @@ def test_full_size_invariants_config3():
     removed, every particle inside its cell, div+B unchanged."""
     import allscale_ipic3d_amd as G
     n = (128, 128, 128)
-    B, c = _dipole_centres(n, radius=4.0)
+    init, c = _dipole_init(n, radius=4.0)
     x, y, z = np.meshgrid(*[np.arange(k) + 0.5 for k in n], indexing="ij")
     r2 = ((x - c[0]) ** 2 + (y - c[1]) ** 2 + (z - c[2]) ** 2).reshape(-1)
     ppc = np.rint(30 * (1 + 9 * np.exp(-r2 / 400.0))).astype(np.int32)
     q = -1.0 / (30 * 4 * np.pi)
     sim = G.Simulation(n, dt=0.1, species=((q, -1.0),), sink_center=tuple(c), sink_radius=4.0)
     sim.init_particles(0, ppc, (0.05,) * 3, drift=(0.1, 0.0, 0.0), seed=3)
-    sim.upload_fields(np.zeros((n[0] * n[1] * n[2], 3)), B)
+    sim.init_fields(init)
     total = int(ppc.sum())
     print("particles", total, "ppc", int(ppc.min()), int(ppc.max()), flush=True)
     assert ppc.min() == 30 and ppc.max() >= 299 and total > 70_000_000
