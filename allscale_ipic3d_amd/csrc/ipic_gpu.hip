@@ -185,6 +185,17 @@ __device__ __forceinline__ int wave_incl_scan(int x, int) {
     return x;
 }
 
+// Inclusive prefix maximum over the wave of non-negative values, with the same lane moves.
+__device__ __forceinline__ int wave_incl_max(int x) {
+    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, true));  // row_shr:1
+    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, true));  // row_shr:2
+    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, true));  // row_shr:4
+    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, true));  // row_shr:8
+    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false));  // row_bcast:15 into rows 1, 3
+    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false));  // row_bcast:31 into rows 2, 3
+    return x;
+}
+
 // Exclusive prefix sum over the block (NT threads); returns the prefix, sets total.
 __device__ __forceinline__ int block_exscan(int v, int* s_w, int& total) {
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -208,17 +219,18 @@ __device__ __forceinline__ int block_exscan(int v, int* s_w, int& total) {
 // independent of the order of the adds.  The conversion rounds v * 2^fix_bits to the nearest
 // integer x in one FP64 fused multiply-add with 1.5 * 2^52, whose unit in the last place is 1:
 // for |x| < 2^51 the bit pattern of the result is bits(1.5 * 2^52) + x, and as the constant's
-// low word is zero, x is that pattern with the constant's high word subtracted.  v is clamped
-// to |x| <= 2^50 first (NaN to the lower bound), so the identity always holds; the callers
-// raise ERR_JSAT when a value reaches the clamp.
-__device__ __forceinline__ long long fix_of(float v, float lim, double scale) {
-    const float vc = fminf(fmaxf(v, -lim), lim);
-    const double d = fma((double)vc, scale, 6755399441055744.0);
+// low word is zero, x is that pattern with the constant's high word subtracted.  The callers
+// keep |v| <= fix_lim = 2^(50 - fix_bits) (|x| <= 2^50) or raise ERR_JSAT: every deposited
+// value is a sum of at most run_in terms q v w with weights in [0, 1], checked against the
+// run's largest |q v| component, and velocities are finite (a non-finite position removes the
+// particle; uploads refuse non-finite values).
+__device__ __forceinline__ long long fix_of(float v, double scale) {
+    const double d = fma((double)v, scale, 6755399441055744.0);
     return (long long)((unsigned long long)__double_as_longlong(d) - 0x4338000000000000ull);
 }
 
-__device__ __forceinline__ void fix_add(long long* p, float v, float lim, double scale) {
-    __hip_atomic_fetch_add(p, fix_of(v, lim, scale), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+__device__ __forceinline__ void fix_add(long long* p, float v, double scale) {
+    __hip_atomic_fetch_add(p, fix_of(v, scale), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
 // device error bits (the host turns them into an error of the next synchronising call)
@@ -279,6 +291,8 @@ __device__ __forceinline__ void deposit_direct(const PushArgs& A, const TG& G, l
                                                int z0, int elx, int ely, int elz, const float wt[8], float jx,
                                                float jy, float jz) {
     const Geo& g = A.g;
+    // every corner value is at most max |j| (the weights lie in [0, 1])
+    if (!(fmaxf(fabsf(jx), fmaxf(fabsf(jy), fabsf(jz))) <= g.fix_lim)) atomicOr(A.err, ERR_JSAT);
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -289,26 +303,24 @@ __device__ __forceinline__ void deposit_direct(const PushArgs& A, const TG& G, l
                 const int nx = elx + JH + i, ny = ely + JH + j, nz = elz + JH + k;
                 if (nx >= 0 && nx < G.nj0 && ny >= 0 && ny < G.nj1 && nz >= 0 && nz < G.nj2) {
                     const int l = (nx * G.nj1 + ny) * G.nj2 + nz;
-                    fix_add(&sJ[l], jx * ww, g.fix_lim, g.fix_scale);
-                    fix_add(&sJ[G.NJ + l], jy * ww, g.fix_lim, g.fix_scale);
-                    fix_add(&sJ[2 * G.NJ + l], jz * ww, g.fix_lim, g.fix_scale);
+                    fix_add(&sJ[l], jx * ww, g.fix_scale);
+                    fix_add(&sJ[G.NJ + l], jy * ww, g.fix_scale);
+                    fix_add(&sJ[2 * G.NJ + l], jz * ww, g.fix_scale);
                 } else {
                     // beyond the tile: the global accumulator, in the same fixed point, so J does
                     // not depend on the order of these adds either
                     const int gid = nid_of(g, wrapc(x0 + elx + i, g.n[0]), wrapc(y0 + ely + j, g.n[1]),
                                            wrapc(z0 + elz + k, g.n[2]));
                     long long* o = A.jover + 3 * (long long)gid;
-                    __hip_atomic_fetch_add(o + 0, fix_of(jx * ww, g.fix_lim, g.fix_scale), __ATOMIC_RELAXED,
+                    __hip_atomic_fetch_add(o + 0, fix_of(jx * ww, g.fix_scale), __ATOMIC_RELAXED,
                                            __HIP_MEMORY_SCOPE_AGENT);
-                    __hip_atomic_fetch_add(o + 1, fix_of(jy * ww, g.fix_lim, g.fix_scale), __ATOMIC_RELAXED,
+                    __hip_atomic_fetch_add(o + 1, fix_of(jy * ww, g.fix_scale), __ATOMIC_RELAXED,
                                            __HIP_MEMORY_SCOPE_AGENT);
-                    __hip_atomic_fetch_add(o + 2, fix_of(jz * ww, g.fix_lim, g.fix_scale), __ATOMIC_RELAXED,
+                    __hip_atomic_fetch_add(o + 2, fix_of(jz * ww, g.fix_scale), __ATOMIC_RELAXED,
                                            __HIP_MEMORY_SCOPE_AGENT);
                     *A.jflag = 1;
                 }
             }
-    // every corner value is at most max |j| (the weights lie in [0, 1])
-    if (fmaxf(fabsf(jx), fmaxf(fabsf(jy), fabsf(jz))) > g.fix_lim) atomicOr(A.err, ERR_JSAT);
 }
 
 template <class TG>
@@ -417,7 +429,7 @@ __device__ __forceinline__ void boris_move(const Geo& g, float qom, float subk, 
     const float f = 2.0f * __builtin_amdgcn_rcpf(1.0f + (tx * tx + ty * ty + tz * tz));
     const float ux = f * tx, uy = f * ty, uz = f * tz;
     const float kex = kq * Ex, key = kq * Ey, kez = kq * Ez;
-    for (int sc = 0; sc < nsub; ++sc) {
+    auto sub = [&]() {
         const float mx = vx + kex, my = vy + key, mz = vz + kez;
         const float qx = mx + (my * tz - mz * ty);
         const float qy = my + (mz * tx - mx * tz);
@@ -428,7 +440,9 @@ __device__ __forceinline__ void boris_move(const Geo& g, float qom, float subk, 
         px += vx * sx;
         py += vy * sy;
         pz += vz * sz;
-    }
+    };
+    sub();  // the one sub-cycle of the common case outside the loop
+    for (int sc = 1; sc < nsub; ++sc) sub();
 }
 
 // Reflection at the global faces (cell.h:1103-1121) for a particle of cell (cx,cy,cz) at offsets
@@ -519,6 +533,38 @@ __device__ __forceinline__ void stp(Rsrc r, float v, unsigned voff, unsigned sof
     __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, voff, soff, 0);
 }
 
+// the first n (1 <= n <= N) of v to n consecutive floats: dwordx4 stores, then the rest in one
+template <int N>
+__device__ __forceinline__ void st_run(Rsrc r, unsigned voff, unsigned soff, const float (&v)[N], int n) {
+    typedef unsigned u4 __attribute__((ext_vector_type(4)));
+    typedef unsigned u3 __attribute__((ext_vector_type(3)));
+    typedef unsigned u2 __attribute__((ext_vector_type(2)));
+    int i = 0;
+#pragma unroll
+    for (int b = 0; b + 4 <= N; b += 4) {
+        if (n - b >= 4) {
+            const u4 q = {__float_as_uint(v[b]), __float_as_uint(v[b + 1]), __float_as_uint(v[b + 2]),
+                          __float_as_uint(v[b + 3])};
+            __builtin_amdgcn_raw_buffer_store_b128(q, r, voff + 4u * b, soff, 0);
+            i = b + 4;
+        }
+    }
+#pragma unroll
+    for (int b = 0; b < N; b += 4) {
+        if (b != i) continue;  // the block of the remaining floats
+        const int m = n - b;
+        if (m == 1) {
+            stp(r, v[b], voff + 4u * b, soff);
+        } else if (m == 2 && b + 1 < N) {
+            const u2 q = {__float_as_uint(v[b]), __float_as_uint(v[b + 1])};
+            __builtin_amdgcn_raw_buffer_store_b64(q, r, voff + 4u * b, soff, 0);
+        } else if (m == 3 && b + 2 < N) {
+            const u3 q = {__float_as_uint(v[b]), __float_as_uint(v[b + 1]), __float_as_uint(v[b + 2])};
+            __builtin_amdgcn_raw_buffer_store_b96(q, r, voff + 4u * b, soff, 0);
+        }
+    }
+}
+
 __device__ __forceinline__ void store_particle(Rsrc r, unsigned cs, int oidx, float px, float py, float pz,
                                                float vx, float vy, float vz) {
     const unsigned vo = (unsigned)oidx * 4u;
@@ -584,8 +630,9 @@ __host__ __device__ inline PLayout particle_layout(int NJ, int TC, int NF) {
     L.off_wlo = o; o += (NW + 1) * 4;
     o = (o + 15) & ~15;
     L.off_wave = o;
-    // per wave: run table (WS+1 ints), arrival queue (QCAP ints), leaver buffer (K0CAP x 8 floats)
-    L.wave_bytes = ((WS + 1) * 4 + QCAP * 4 + 15) & ~15;
+    // per wave: run table (WS+1 ints), run heads (WS ints), arrival queue (QCAP ints), leaver
+    // buffer (K0CAP x 8 floats)
+    L.wave_bytes = ((WS + 1) * 4 + WS * 4 + QCAP * 4 + 15) & ~15;
     L.wave_bytes += K0CAP * 8 * 4;
     o += NW * L.wave_bytes;
     L.total = o;
@@ -628,8 +675,9 @@ __global__ __launch_bounds__(NTP) IPIC_PUSH_ATTR void k_particles(PushArgs A) {
     const int tid = threadIdx.x, lane = tid & (WS - 1), wid = tid / WS;
     unsigned char* wbase = smem + L.off_wave + wid * L.wave_bytes;
     int* R = (int*)wbase;                                            // WS+1 ints
-    int* Q = (int*)(wbase + (WS + 1) * 4);                           // arrival queue
-    float* K0 = (float*)(wbase + (((WS + 1) * 4 + QCAP * 4 + 15) & ~15));  // leaver buffer
+    int* H = (int*)(wbase + (WS + 1) * 4);                           // WS ints
+    int* Q = (int*)(wbase + (2 * WS + 1) * 4);                       // arrival queue
+    float* K0 = (float*)(wbase + (((2 * WS + 1) * 4 + QCAP * 4 + 15) & ~15));  // leaver buffer
 
     const int t = A.tile_order ? A.tile_order[blockIdx.x] : xcd_tile(blockIdx.x, g.ntiles);
     const int sl = blockIdx.y;  // this block's slice of the tile's cells
@@ -802,6 +850,11 @@ __global__ __launch_bounds__(NTP) IPIC_PUSH_ATTR void k_particles(PushArgs A) {
             const int incl = wave_incl_scan(rl, lane);
             R[lane + 1] = incl;
             if (lane == 0) R[0] = 0;
+            // run heads: H[r] = the cell whose first run is run r (0 elsewhere); the cell of run r
+            // is then the running maximum of H up to r
+            H[lane] = 0;
+            wave_sync();
+            if (rl > 0 && incl - rl < WS) H[incl - rl] = lane;
             const int all_runs = __shfl(incl, WS - 1, WS);
             wave_sync();
             const int nruns = min(all_runs, WS);
@@ -812,14 +865,8 @@ __global__ __launch_bounds__(NTP) IPIC_PUSH_ATTR void k_particles(PushArgs A) {
                 continue;
             }
             const bool active = lane < nruns;
-            int k = 0, len = 0, pst = 0;
+            int k = wave_incl_max(H[lane]), len = 0, pst = 0;  // the run's cell: c_cur + k
             if (active) {
-                int lo = 0, hi = WS - 1;  // largest k with R[k] <= lane
-                while (lo < hi) {
-                    const int mid = (lo + hi + 1) >> 1;
-                    if (R[mid] <= lane) lo = mid; else hi = mid - 1;
-                }
-                k = lo;
                 const int j = lane - R[k];
                 const int nr = R[k + 1] - R[k];
                 const int p0 = (k == 0) ? pos_cur : s_seg[c_cur + k];
@@ -844,6 +891,8 @@ __global__ __launch_bounds__(NTP) IPIC_PUSH_ATTR void k_particles(PushArgs A) {
             float rx[KRUN], ry[KRUN], rz[KRUN], rvx[KRUN], rvy[KRUN], rvz[KRUN];
             uint32_t rid[TRACK ? KRUN : 1];
             const unsigned vrun = (unsigned)pst * 4u;
+            // (dword loads: wider loads need aligned register tuples that push the kernel past its
+            // register budget)
 #pragma unroll
             for (int jj = 0; jj < KRUN; ++jj) {
                 if (jj < len) {
@@ -940,6 +989,27 @@ __global__ __launch_bounds__(NTP) IPIC_PUSH_ATTR void k_particles(PushArgs A) {
 #pragma unroll
             for (int m = 0; m < 8; ++m) mom[0][m] = mom[1][m] = mom[2][m] = 0.0f;
             float jmax = 0.0f;  // largest |j| component of the run: bounds every corner sum / run_in
+            // a run of home particles only (the common case) takes consecutive slots: its six
+            // arrays go out in a few stores covering whole lines across the wave
+            const bool whole = (MODE != MODE_DEPOSIT) && len > 0 && hm == (1u << len) - 1u;
+            if (whole) {
+                const int o0 = a_start + base;
+                if (o0 < 0 || o0 + len > a_lim) {
+                    atomicOr(A.err, ERR_SLOT);
+                } else {
+                    const unsigned vo = (unsigned)o0 * 4u;
+                    st_run(rout, vo, 0, rx, len);
+                    st_run(rout, vo, cs, ry, len);
+                    st_run(rout, vo, 2 * cs, rz, len);
+                    st_run(rout, vo, 3 * cs, rvx, len);
+                    st_run(rout, vo, 4 * cs, rvy, len);
+                    st_run(rout, vo, 5 * cs, rvz, len);
+                    if (TRACK)
+#pragma unroll
+                        for (int jj = 0; jj < KRUN; ++jj)
+                            if (jj < len) S.id_out[o0 + jj] = rid[jj];
+                }
+            }
 #pragma unroll
             for (int jj = 0; jj < KRUN; ++jj) {
                 int qe = -1;       // arrival queue entry of this lane
@@ -951,7 +1021,7 @@ __global__ __launch_bounds__(NTP) IPIC_PUSH_ATTR void k_particles(PushArgs A) {
                 if (home) {
                     // a home particle's x offset is NaN only when pass B1 removed it
                     const bool sunk = (MODE == MODE_PUSH) && __builtin_isnan(px);
-                    if (MODE != MODE_DEPOSIT) {
+                    if (MODE != MODE_DEPOSIT && !whole) {
                         const int oidx = a_start + base + rank;
                         ++rank;
                         if ((unsigned)oidx >= (unsigned)a_lim) {
@@ -1005,7 +1075,7 @@ __global__ __launch_bounds__(NTP) IPIC_PUSH_ATTR void k_particles(PushArgs A) {
                 kn += __popcll(bk);
             }
             if (MODE != MODE_REBIN && run_in > 0) {
-                if (jmax * (float)run_in > g.fix_lim) atomicOr(A.err, ERR_JSAT);
+                if (!(jmax * (float)run_in <= g.fix_lim)) atomicOr(A.err, ERR_JSAT);
 #pragma unroll
                 for (int d = 0; d < 3; ++d) {
                     float N[8];
@@ -1017,7 +1087,7 @@ __global__ __launch_bounds__(NTP) IPIC_PUSH_ATTR void k_particles(PushArgs A) {
 #pragma unroll
                             for (int kk = 0; kk < 2; ++kk) {
                                 const int l = ((lx + JH + i) * G.nj1 + (ly + JH + j)) * G.nj2 + (lz + JH + kk);
-                                fix_add(&sJ[d * G.NJ + l], N[(i * 2 + j) * 2 + kk], g.fix_lim, g.fix_scale);
+                                fix_add(&sJ[d * G.NJ + l], N[(i * 2 + j) * 2 + kk], g.fix_scale);
                             }
                 }
             }
@@ -2470,7 +2540,9 @@ int ipic_gpu_upload_particles(ipic_ctx* ctx, int32_t species, int64_t n, const d
                 if (o > 1.0f) o = 1.0f;
                 off[3 * i + k] = o;
             }
-            cell[i] = host_cid(g, ci[0], ci[1], ci[2]);
+            for (int k = 0; k < 3; ++k)
+            if (!std::isfinite(vel[3 * i + k])) return fail(ctx, IPIC_E_ARG, "non-finite particle velocity");
+        cell[i] = host_cid(g, ci[0], ci[1], ci[2]);
             count[cell[i] + 1]++;
         }
         std::vector<int> start(nc + 1, 0);

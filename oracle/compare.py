@@ -75,12 +75,16 @@ def field_errors(u, sim, names=("E", "Bc", "Bn", "J")):
     return {k: rel_l2(got[k], ref[k]) for k in names}
 
 
-def conservation_errors(u, sim):
+def conservation_errors(u, sim, species):
+    """Relative errors of the FP64 sums; the charge relative to the summed magnitude of the
+    species' charges (the total of a neutral plasma is zero up to rounding)."""
     c = sim.conservation()
     wE, wB = u.field_energy()
-    ref = {"kinetic_energy": u.kinetic_energy(), "momentum": u.momentum(), "charge": u.total_charge(),
-           "field_energy_E": wE, "field_energy_B": wB}
+    ref = {"kinetic_energy": u.kinetic_energy(), "momentum": u.momentum(), "field_energy_E": wE,
+           "field_energy_B": wB}
     out = {k: abs(c[k] - v) / abs(v) if v != 0 else abs(c[k]) for k, v in ref.items()}
+    qabs = sum(abs(q) * sim.num_particles(s) for s, (q, _) in enumerate(species))
+    out["charge"] = abs(c["charge"] - u.total_charge()) / qabs if qabs > 0 else abs(c["charge"])
     out["particles"] = int(c["particles"]) - u.num_particles()
     return out
 
@@ -89,4 +93,4 @@ def compare(u, sim, species, w=(1.0, 1.0, 1.0), origin=(0.0, 0.0, 0.0)):
     """Every comparison of a GPU state with the oracle's: fields, per-cell particle sums,
     conservation sums."""
     return {"fields": field_errors(u, sim), "particles": particle_errors(u, sim, species, w, origin),
-            "conservation": conservation_errors(u, sim)}
+            "conservation": conservation_errors(u, sim, species)}
