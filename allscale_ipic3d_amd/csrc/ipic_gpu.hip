@@ -1162,6 +1162,7 @@ struct FieldArgs {
     long long* jover;
     int* jflag;
     const int* jpairs;  // [3][nmax][1+JPMAX]: count, then (tile << 8 | local)
+    int nsl_read;       // slices of each tile's partials the sum reads: 1 once k_slice_sum ran
     float dt;           // field substep
     float c2;           // c^2 (field energies)
     float eF;           // factor of curl- B in the E pass: c^2 (leapfrog) or c (forward solver)
@@ -1207,7 +1208,7 @@ __device__ __forceinline__ float4 sum_partials(const FieldArgs& F, int i, int x,
                 const int tile = (ta * g.NTl[1] + tb) * g.NTl[2] + tc;
                 const int l = (la * g.nj[1] + lb) * g.nj[2] + lc;
                 const float* base = F.jpart + (long long)tile * g.nsl * 3 * g.NJ;
-                for (int s = 0; s < g.nsl; ++s, base += 3 * g.NJ) {
+                for (int s = 0; s < F.nsl_read; ++s, base += 3 * g.NJ) {
                     sx += (double)base[l];
                     sy += (double)base[g.NJ + l];
                     sz += (double)base[2 * g.NJ + l];
@@ -1223,6 +1224,20 @@ __device__ __forceinline__ float4 sum_partials(const FieldArgs& F, int i, int x,
         o[2] = 0;
     }
     return make_float4((float)sx, (float)sy, (float)sz, 0.0f);
+}
+
+// The block slices of a tile (grids of few tiles split each tile among nsl blocks) summed in
+// FP64 into the tile's first slice, one thread per partial entry: the node sums then read one
+// slice per tile instead of nsl.
+__global__ __launch_bounds__(256) void k_slice_sum(Geo g, float* jpart) {
+    const long long per = 3LL * g.NJ;
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long long)g.ntiles * per) return;
+    const long long t = i / per, e = i - t * per;
+    float* base = jpart + t * g.nsl * per + e;
+    double acc = 0.0;
+    for (int s = 0; s < g.nsl; ++s) acc += (double)base[s * per];
+    base[0] = (float)acc;
 }
 
 // J from the tile partials, without a field update (ahead of the field operator).
@@ -1898,6 +1913,7 @@ FieldArgs field_args(ipic_ctx* ctx) {
     F.jover = ctx->jover;
     F.jflag = ctx->jflag;
     F.jpairs = ctx->jpairs;
+    F.nsl_read = ctx->g.nsl;
     F.dt = ctx->d.field_substeps > 0 ? (float)(ctx->d.dt / ctx->d.field_substeps) : 0.0f;
     F.c2 = (float)(ctx->d.c * ctx->d.c);
     const bool fwd = (ctx->d.flags & IPIC_FLAG_FORWARD_SOLVER) != 0;
@@ -1989,15 +2005,30 @@ int launch_scan(ipic_ctx* ctx) {
 
 int launch_jsum(ipic_ctx* ctx);
 
+// Reduces the block slices of the partials (a no-op with one slice per tile); F reads one slice
+// afterwards.
+int slice_sum(ipic_ctx* ctx, FieldArgs& F) {
+    const Geo& g = ctx->g;
+    if (g.nsl > 1) {
+        const long long n = (long long)g.ntiles * 3 * g.NJ;
+        hipLaunchKernelGGL(k_slice_sum, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, g, ctx->jpart);
+        HIPCK(ctx, hipGetLastError());
+    }
+    F.nsl_read = 1;
+    return 0;
+}
+
 int launch_leapfrog(ipic_ctx* ctx) {
-    // the static field (no substeps) still forms J from the deposit, so J and the overflow
-    // accumulator describe the current state as after a leapfrog phase
-    if (ctx->d.field_substeps == 0) return launch_jsum(ctx);
+    // the static field uses no J: it is formed on demand (download, conservation, field state)
+    // from the current deposit, and the next deposit pass clears the overflow accumulator
+    if (ctx->d.field_substeps == 0) return 0;
     FieldArgs F = field_args(ctx);
     const int nb = (ctx->g.ncell + 255) / 256;
     for (int s = 0; s < ctx->d.field_substeps; ++s) {
         // the first E pass forms J from the partials unless that was done already
         const int from_partials = (s == 0 && !ctx->j_summed) ? 1 : 0;
+        if (from_partials)
+            if (int rc = slice_sum(ctx, F)) return rc;
         hipLaunchKernelGGL(k_field_E, dim3(nb), dim3(256), 0, ctx->stream, F, from_partials);
         hipLaunchKernelGGL(k_field_B, dim3(nb), dim3(256), 0, ctx->stream, F);
     }
@@ -2019,6 +2050,7 @@ int launch_c2n(ipic_ctx* ctx) {
 int launch_jsum(ipic_ctx* ctx) {
     if (ctx->j_summed) return 0;
     FieldArgs F = field_args(ctx);
+    if (int rc = slice_sum(ctx, F)) return rc;
     const int nb = (ctx->g.ncell + 255) / 256;
     hipLaunchKernelGGL(k_jsum, dim3(nb), dim3(256), 0, ctx->stream, F);
     hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(1), 0, ctx->stream, ctx->jflag);
@@ -3013,9 +3045,22 @@ int ipic_gpu_download_particles(ipic_ctx* ctx, int32_t species, double* pos, dou
     });
 }
 
+// With the static field no field update uses J: it is the current density of the current
+// particle state, formed here from the deposit when it is read.
+int static_field_j(ipic_ctx* ctx) {
+    if (ctx->d.field_substeps != 0) return 0;
+    if (!ctx->j_valid) {
+        if (int rc = launch_particles(ctx, MODE_DEPOSIT)) return rc;
+        ctx->j_valid = true;
+    }
+    return launch_jsum(ctx);
+}
+
 int ipic_gpu_download_fields(ipic_ctx* ctx, double* E, double* Bc, double* Bn, double* J) {
     return guarded(ctx, [&]() -> int {
         if (int rc = check_ctx(ctx)) return rc;
+        if (J)
+            if (int rc = static_field_j(ctx)) return rc;
         const Geo& g = ctx->g;
         const int nc = g.ncell;
         std::vector<double> tB;
@@ -3084,6 +3129,7 @@ int ipic_gpu_conservation(ipic_ctx* ctx, ipic_conservation* out) {
             r.charge += cnt * S.q;
             r.particles += (int64_t)cnt;
         }
+        if (int rc = static_field_j(ctx)) return rc;
         FieldArgs F = field_args(ctx);
         hipLaunchKernelGGL(k_reduce_fields, dim3(ctx->red_blocks), dim3(256), 0, ctx->stream, F, ctx->red);
         HIPCK(ctx, hipGetLastError());

@@ -164,7 +164,8 @@ int ipic_gpu_download_particles(ipic_ctx* ctx, int32_t species, double* pos, dou
                                 int64_t* cell, uint32_t* ids);
 
 /* Field state; any pointer may be NULL.  J is the current density used by the last
- * field update. */
+ * field update; with the static field (field_substeps = 0, no update uses J) the current
+ * density of the current particle state. */
 int ipic_gpu_download_fields(ipic_ctx* ctx, double* E, double* Bc, double* Bn, double* J);
 
 /* FP64 conservation sums of the current state. */

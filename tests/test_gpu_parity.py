@@ -159,10 +159,10 @@ def test_benchmarked_instance_two_species_16c_32ppc():
 
 
 def test_static_field_keeps_j_of_the_current_deposit():
-    """Static field (field_substeps = 0, field.h:272-284): J is still formed from each step's
-    deposit, so it matches the oracle after step(n) and after a particle step followed by a
-    field-state read; deposits beyond a tile (fast particles on a 3^3 grid, one tile) do not
-    accumulate across steps."""
+    """Static field (field_substeps = 0, field.h:272-284): no update uses J, so J is the current
+    density of the current particles, formed when it is read.  It matches the oracle's deposit
+    after step(n) and after a particle step followed by a field-state read; deposits beyond a
+    tile (fast particles on a 3^3 grid, one tile) do not accumulate across steps."""
     rng = np.random.default_rng(13)
     n = (3, 3, 3)
     u = O.Universe(n, dt=0.5, c=1.0, field_substeps=0)
@@ -171,19 +171,29 @@ def test_static_field_keeps_j_of_the_current_deposit():
     vel = rng.uniform(-2.5, 2.5, size=(N, 3)) * 0.3
     u.add_particles(O.make_particles(pos, vel, 0.01, -1.0))
     Bc = np.tile([0.0, 0.0, 0.3], (27, 1))
-    u.set_fields(np.zeros((27, 3)), Bc)
+    E = rng.normal(0.0, 0.01, size=(27, 3))
+    u.set_fields(E, Bc)
     import allscale_ipic3d_amd as G
     sim = G.Simulation(n, dt=0.5, species=((0.01, -1.0),), field_substeps=0)
     sim.upload_particles(0, pos, vel)
-    sim.upload_fields(np.zeros((27, 3)), Bc)
+    sim.upload_fields(E, Bc)
     u.step(3)
     sim.step(3)
+    u.deposit()
     _compare_fields(u, sim)
     # one more particle phase: the J of the new state, formed on demand
     u.move(); u.rebin(); u.deposit()
     sim.particle_step()
     sim.field_state()
     _compare_fields(u, sim, names=("J",))
+    # and through the conservation sums (E . J)
+    sim.particle_step()
+    u.move(); u.rebin(); u.deposit()
+    cu = sim.conservation()
+    f = u.fields()
+    ej = float(np.sum(f["E"] * f["J"]))
+    assert abs(ej) > 1e-6
+    assert cu["EdotJ"] == pytest.approx(ej, rel=1e-5)
 
 
 def test_fields_only_leapfrog():
