@@ -22,6 +22,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+
 #include <cmath>
 #include <cstdint>
 #include <string>
@@ -55,9 +57,16 @@ __host__ __device__ constexpr long long loff(int l) { return l == 0 ? 0 : loff(l
 constexpr long long NPARAM = loff(NL);
 
 thread_local std::string g_err;
+thread_local const char* g_err_lit = nullptr;  // message of a failure that allocates nothing
 
 int fail(int code, const std::string& m) {
+    g_err_lit = nullptr;
     g_err = m;
+    return code;
+}
+
+int fail_lit(int code, const char* m) {
+    g_err_lit = m;
     return code;
 }
 
@@ -597,11 +606,37 @@ HeadArgs head_args(const int32_t n[3], const float* out, const ipic_nop_scales* 
     return h;
 }
 
+// Scratch of ipic_nop_apply_noalloc: the head's potential psi for grids of at most
+// IPIC_NOP_NOALLOC_MAX_VOXELS voxels (32 MiB of float4), one copy per GPU the library runs on,
+// held until the process ends.  Host code uses the per-device address from
+// hipGetSymbolAddress, never the array's name (a host-side stand-in).
+static __device__ __align__(256) float4 scratch[IPIC_NOP_NOALLOC_MAX_VOXELS];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::atomic<float4*> g_scratch_ptr[NOALLOC_MAX_DEVICES];
+
+// The current device's copy of `scratch`, looked up on the first call on that device.
+int noalloc_scratch(float4** out) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return fail_lit(IPIC_E_HIP, "hipGetDevice failed");
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return fail_lit(IPIC_E_RANGE, "device index above 63");
+    float4* p = g_scratch_ptr[dev].load(std::memory_order_acquire);
+    if (!p) {
+        void* q = nullptr;
+        if (hipGetSymbolAddress(&q, HIP_SYMBOL(scratch)) != hipSuccess || !q)
+            return fail_lit(IPIC_E_HIP, "hipGetSymbolAddress of the scratch array failed");
+        p = (float4*)q;
+        g_scratch_ptr[dev].store(p, std::memory_order_release);
+    }
+    *out = p;
+    return 0;
+}
+
 }  // namespace
 
 extern "C" {
 
-const char* ipic_nop_last_error(void) { return g_err.c_str(); }
+const char* ipic_nop_last_error(void) { return g_err_lit ? g_err_lit : g_err.c_str(); }
 
 int64_t ipic_nop_param_count(void) { return NPARAM; }
 
@@ -675,6 +710,28 @@ int ipic_nop_apply(const int32_t n[3], const float* out, const ipic_nop_scales* 
     hipLaunchKernelGGL(k_head_a, g, dim3(256), 0, (hipStream_t)stream, h, 0);
     hipLaunchKernelGGL(k_head_b, g, dim3(256), 0, (hipStream_t)stream, h, 0);
     NOPCK(hipGetLastError());
+    return 0;
+}
+
+int ipic_nop_apply_noalloc(const int32_t n[3], const float* out, const ipic_nop_scales* s, const float inv_w[3],
+                           float* dE4, float* dBc4, void* stream) {
+    if (!n || n[0] <= 0 || n[1] <= 0 || n[2] <= 0 || n[0] % BX || n[1] % BY || n[2] % BZ)
+        return fail_lit(IPIC_E_ARG, "grid must be positive with nx % 4 == 0, ny % 4 == 0, nz % 16 == 0");
+    const long long V = voxels(n);
+    if (V > IPIC_NOP_NOALLOC_MAX_VOXELS)
+        return fail_lit(IPIC_E_RANGE, "grid above IPIC_NOP_NOALLOC_MAX_VOXELS (2097152 voxels) for the "
+                                      "no-allocation apply");
+    if (!out || !s || !dE4 || !dBc4) return fail_lit(IPIC_E_ARG, "null pointer");
+    float4* psi = nullptr;
+    if (int rc = noalloc_scratch(&psi)) return rc;
+    HeadArgs h = head_args(n, out, s, inv_w);
+    h.E = (float4*)dE4;
+    h.B = (float4*)dBc4;
+    h.psi = psi;  // every entry below V is written by k_head_a before k_head_b reads it
+    const dim3 g((unsigned)((V + 255) / 256));
+    hipLaunchKernelGGL(k_head_a, g, dim3(256), 0, (hipStream_t)stream, h, 0);
+    hipLaunchKernelGGL(k_head_b, g, dim3(256), 0, (hipStream_t)stream, h, 0);
+    if (hipGetLastError() != hipSuccess) return fail_lit(IPIC_E_HIP, "kernel launch failed");
     return 0;
 }
 

@@ -20,7 +20,7 @@ NOP_SYMBOLS = (
     "ipic_nop_last_error", "ipic_nop_param_count", "ipic_nop_layer", "ipic_nop_workspace_bytes",
     "ipic_nop_prepare", "ipic_nop_pack", "ipic_nop_forward", "ipic_nop_apply", "ipic_nop_loss",
     "ipic_nop_backward", "ipic_nop_adam", "ipic_dev_alloc", "ipic_dev_free", "ipic_dev_copy",
-    "ipic_dev_zero", "ipic_dev_sync", "ipic_dev_sub4",
+    "ipic_dev_zero", "ipic_dev_sync", "ipic_dev_sub4", "ipic_nop_apply_noalloc",
 )
 
 LAYERS = [(16, 32)] + [(32, 32)] * 8 + [(32, 16)]
@@ -57,6 +57,7 @@ def nop_lib():
     L.ipic_nop_pack.argtypes = [N3, P, P, P, S, P, P]
     L.ipic_nop_forward.argtypes = [N3, P, P, P, P, I32, P, P]
     L.ipic_nop_apply.argtypes = [N3, P, S, F3, P, P, P, P]
+    L.ipic_nop_apply_noalloc.argtypes = [N3, P, S, F3, P, P, P]
     L.ipic_nop_loss.argtypes = [N3, P, S, F3, P, P, F32, F32, P, P, P, P, P]
     L.ipic_nop_backward.argtypes = [N3, P, P, P, P, P, P, P]
     L.ipic_nop_adam.argtypes = [P, P, P, P, I64, F32, F32, F32, F32, P, P, P]

@@ -73,6 +73,19 @@ int ipic_nop_forward(const int32_t n[3], const uint16_t* u, const uint16_t* wf, 
 int ipic_nop_apply(const int32_t n[3], const float* out, const ipic_nop_scales* s, const float inv_w[3],
                    float* dE4, float* dBc4, float* psi4, void* stream);
 
+/* ipic_nop_apply without the psi4 workspace, for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops); it has no counterpart in the reference.
+ * The potential lives in a file-scope device array of the library, IPIC_NOP_NOALLOC_MAX_VOXELS
+ * float4 (32 MiB) on each GPU the library runs on, held until the process ends.  Allocates
+ * nothing: no hipMalloc, hipFree or allocating library call, and its error messages are static
+ * strings.  The first call on a device looks the array up (hipGetSymbolAddress): make it
+ * before any capture.  Every call on one GPU shares that array, so calls that overlap on one
+ * GPU (concurrent streams) race; each call writes every entry it reads.  Grids above the
+ * maximum are refused with IPIC_E_RANGE. */
+#define IPIC_NOP_NOALLOC_MAX_VOXELS 2097152
+int ipic_nop_apply_noalloc(const int32_t n[3], const float* out, const ipic_nop_scales* s, const float inv_w[3],
+                           float* dE4, float* dBc4, void* stream);
+
 /* Training loss L = mean over i of |dE_pred - tE|^2 / varE + |dB_pred - tB|^2 / varB, where
  * (dE_pred, dB_pred) are the increments ipic_nop_apply would make and tE, tB (n^3 float4)
  * the targets.  Adds L to *loss (one FP64 on the device) and writes dout = dL/do

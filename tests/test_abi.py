@@ -14,7 +14,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def _declared(header):
     txt = open(os.path.join(ROOT, "include", header)).read()
-    return sorted(set(re.findall(r"^(?:int|const char\*)\s+(ipic_\w+)\s*\(", txt, flags=re.M)))
+    return sorted(set(re.findall(r"^(?:int|int64_t|const char\*)\s+(ipic_\w+)\s*\(", txt, flags=re.M)))
 
 
 def test_header_declares_binding_symbols():
@@ -25,6 +25,18 @@ def test_header_declares_binding_symbols():
 def test_library_exports_every_declared_symbol():
     lib = ctypes.CDLL(G.LIB_PATH)
     for name in _declared("ipic_gpu.h"):
+        assert hasattr(lib, name), name
+
+
+def test_operator_header_declares_binding_symbols():
+    from allscale_ipic3d_amd import field_operator as FO
+    assert _declared("ipic_nop.h") == sorted(FO.NOP_SYMBOLS)
+
+
+@pytest.mark.skipif(not os.path.exists(G.LIB_PATH), reason="library not built")
+def test_library_exports_every_operator_symbol():
+    lib = ctypes.CDLL(G.LIB_PATH)
+    for name in _declared("ipic_nop.h"):
         assert hasattr(lib, name), name
 
 
