@@ -533,36 +533,12 @@ __device__ __forceinline__ void stp(Rsrc r, float v, unsigned voff, unsigned sof
     __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, voff, soff, 0);
 }
 
-// the first n (1 <= n <= N) of v to n consecutive floats: dwordx4 stores, then the rest in one
+// v[0..3] to four consecutive floats in one dwordx4 store
 template <int N>
-__device__ __forceinline__ void st_run(Rsrc r, unsigned voff, unsigned soff, const float (&v)[N], int n) {
+__device__ __forceinline__ void st4(Rsrc r, unsigned voff, unsigned soff, const float (&v)[N]) {
     typedef unsigned u4 __attribute__((ext_vector_type(4)));
-    typedef unsigned u3 __attribute__((ext_vector_type(3)));
-    typedef unsigned u2 __attribute__((ext_vector_type(2)));
-    int i = 0;
-#pragma unroll
-    for (int b = 0; b + 4 <= N; b += 4) {
-        if (n - b >= 4) {
-            const u4 q = {__float_as_uint(v[b]), __float_as_uint(v[b + 1]), __float_as_uint(v[b + 2]),
-                          __float_as_uint(v[b + 3])};
-            __builtin_amdgcn_raw_buffer_store_b128(q, r, voff + 4u * b, soff, 0);
-            i = b + 4;
-        }
-    }
-#pragma unroll
-    for (int b = 0; b < N; b += 4) {
-        if (b != i) continue;  // the block of the remaining floats
-        const int m = n - b;
-        if (m == 1) {
-            stp(r, v[b], voff + 4u * b, soff);
-        } else if (m == 2 && b + 1 < N) {
-            const u2 q = {__float_as_uint(v[b]), __float_as_uint(v[b + 1])};
-            __builtin_amdgcn_raw_buffer_store_b64(q, r, voff + 4u * b, soff, 0);
-        } else if (m == 3 && b + 2 < N) {
-            const u3 q = {__float_as_uint(v[b]), __float_as_uint(v[b + 1]), __float_as_uint(v[b + 2])};
-            __builtin_amdgcn_raw_buffer_store_b96(q, r, voff + 4u * b, soff, 0);
-        }
-    }
+    const u4 q = {__float_as_uint(v[0]), __float_as_uint(v[1 % N]), __float_as_uint(v[2 % N]), __float_as_uint(v[3 % N])};
+    __builtin_amdgcn_raw_buffer_store_b128(q, r, voff, soff, 0);
 }
 
 __device__ __forceinline__ void store_particle(Rsrc r, unsigned cs, int oidx, float px, float py, float pz,
@@ -998,81 +974,94 @@ __global__ __launch_bounds__(NTP) IPIC_PUSH_ATTR void k_particles(PushArgs A) {
                     atomicOr(A.err, ERR_SLOT);
                 } else {
                     const unsigned vo = (unsigned)o0 * 4u;
-                    st_run(rout, vo, 0, rx, len);
-                    st_run(rout, vo, cs, ry, len);
-                    st_run(rout, vo, 2 * cs, rz, len);
-                    st_run(rout, vo, 3 * cs, rvx, len);
-                    st_run(rout, vo, 4 * cs, rvy, len);
-                    st_run(rout, vo, 5 * cs, rvz, len);
+                    // the first four of each array in one dwordx4 store, the rest one by one
+                    if (KRUN >= 4 && len >= 4) {
+                        st4(rout, vo, 0, rx);
+                        st4(rout, vo, cs, ry);
+                        st4(rout, vo, 2 * cs, rz);
+                        st4(rout, vo, 3 * cs, rvx);
+                        st4(rout, vo, 4 * cs, rvy);
+                        st4(rout, vo, 5 * cs, rvz);
+                    }
+                    const int jlo = (KRUN >= 4 && len >= 4) ? 4 : 0;
+#pragma unroll
+                    for (int jj = 0; jj < KRUN; ++jj)
+                        if (jj >= jlo && jj < len)
+                            store_particle(rout, cs, o0 + jj, rx[jj], ry[jj], rz[jj], rvx[jj], rvy[jj], rvz[jj]);
                     if (TRACK)
 #pragma unroll
                         for (int jj = 0; jj < KRUN; ++jj)
                             if (jj < len) S.id_out[o0 + jj] = rid[jj];
                 }
             }
+            unsigned amask = 0, lmask = 0;  // this run's arrivals (queue entries) and leavers
 #pragma unroll
             for (int jj = 0; jj < KRUN; ++jj) {
-                int qe = -1;       // arrival queue entry of this lane
-                bool leaver = false;
                 const float px = rx[jj], py = ry[jj], pz = rz[jj];
                 const float vx = rvx[jj], vy = rvy[jj], vz = rvz[jj];
                 const bool home = (hm >> jj) & 1u;
-                if (!home && alive(px, py, pz)) qe = pst + jj;  // sits in another cell
-                if (home) {
-                    // a home particle's x offset is NaN only when pass B1 removed it
-                    const bool sunk = (MODE == MODE_PUSH) && __builtin_isnan(px);
-                    if (MODE != MODE_DEPOSIT && !whole) {
-                        const int oidx = a_start + base + rank;
-                        ++rank;
-                        if ((unsigned)oidx >= (unsigned)a_lim) {
-                            atomicOr(A.err, ERR_SLOT);
-                        } else {
-                            store_particle(rout, cs, oidx, px, py, pz, vx, vy, vz);
-                            if (TRACK) S.id_out[oidx] = rid[jj];
-                        }
-                    }
-                    if (MODE == MODE_PUSH && sunk) {
-                        atomicAdd(A.removed, 1ull);
-                    } else if (MODE != MODE_REBIN) {
-                        if (MODE == MODE_DEPOSIT || in_cell(px, py, pz)) {
-                            const float jx = qs * vx, jy = qs * vy, jz = qs * vz;
-                            jmax = fmaxf(jmax, fmaxf(fabsf(jx), fmaxf(fabsf(jy), fabsf(jz))));
-                            const float xy = px * py, xz = px * pz, yz = py * pz, xyz = xy * pz;
-                            const float mono[8] = {1.0f, px, py, pz, xy, xz, yz, xyz};
-#pragma unroll
-                            for (int m = 0; m < 8; ++m) {
-                                mom[0][m] = fmaf(jx, mono[m], mom[0][m]);
-                                mom[1][m] = fmaf(jy, mono[m], mom[1][m]);
-                                mom[2][m] = fmaf(jz, mono[m], mom[2][m]);
-                            }
-                            ++run_in;
-                        } else {
-                            leaver = true;  // leaves cell a: deposit and count after the loop
-                        }
-                    }
-                }
-                // ballot compaction of both queues (every lane executes this)
-                const unsigned long long bq = __ballot(qe >= 0);
-                if (qe >= 0) Q[qn + ballot_rank(bq)] = qe;
-                qn += __popcll(bq);
-                const unsigned long long bk = __ballot(leaver);
-                if (leaver) {
-                    const int slot = kn + ballot_rank(bk);
-                    if (slot < K0CAP) {
-                        float* e = K0 + slot * 8;
-                        e[0] = px; e[1] = py; e[2] = pz; e[3] = vx; e[4] = vy; e[5] = vz;
-                        e[6] = __int_as_float(c);
+                if (!home && alive(px, py, pz)) amask |= 1u << jj;  // sits in another cell
+                if (MODE != MODE_DEPOSIT && !whole && home) {
+                    const int oidx = a_start + base + rank;
+                    ++rank;
+                    if ((unsigned)oidx >= (unsigned)a_lim) {
+                        atomicOr(A.err, ERR_SLOT);
                     } else {
-                        // buffer full: handle in place
-                        const int dex = dcell(px), dey = dcell(py), dez = dcell(pz);
-                        float wt[8];
-                        tri_weights(px - (float)dex, py - (float)dey, pz - (float)dez, wt);
-                        deposit_direct(A, G, sJ, x0, y0, z0, lx + dex, ly + dey, lz + dez, wt, qs * vx, qs * vy,
-                                       qs * vz);
-                        count_add(g, G, s_cnt, S.cnt_acc, x0, y0, z0, ax + dex, ay + dey, az + dez);
+                        store_particle(rout, cs, oidx, px, py, pz, vx, vy, vz);
+                        if (TRACK) S.id_out[oidx] = rid[jj];
                     }
                 }
-                kn += __popcll(bk);
+                // a home particle's x offset is NaN only when pass B1 removed it
+                const bool sunk = (MODE == MODE_PUSH) && home && __builtin_isnan(px);
+                if (sunk) atomicAdd(A.removed, 1ull);
+                if (MODE != MODE_REBIN) {
+                    const bool inc = home && (MODE == MODE_DEPOSIT || in_cell(px, py, pz));
+                    if (home && !sunk && !inc) lmask |= 1u << jj;  // leaves cell a: queued below
+                    // branch-free moments: a particle that deposits elsewhere (or not at all)
+                    // adds zeros
+                    const float dx = inc ? px : 0.0f, dy = inc ? py : 0.0f, dz = inc ? pz : 0.0f;
+                    const float jx = inc ? qs * vx : 0.0f, jy = inc ? qs * vy : 0.0f, jz = inc ? qs * vz : 0.0f;
+                    jmax = fmaxf(jmax, fmaxf(fabsf(jx), fmaxf(fabsf(jy), fabsf(jz))));
+                    const float xy = dx * dy, xz = dx * dz, yz = dy * dz, xyz = xy * dz;
+                    const float mono[8] = {1.0f, dx, dy, dz, xy, xz, yz, xyz};
+#pragma unroll
+                    for (int m = 0; m < 8; ++m) {
+                        mom[0][m] = fmaf(jx, mono[m], mom[0][m]);
+                        mom[1][m] = fmaf(jy, mono[m], mom[1][m]);
+                        mom[2][m] = fmaf(jz, mono[m], mom[2][m]);
+                    }
+                    run_in += inc ? 1 : 0;
+                }
+            }
+            // ballot compaction of both queues, for the chunks that have entries
+            if (__ballot((amask | lmask) != 0)) {
+#pragma unroll
+                for (int jj = 0; jj < KRUN; ++jj) {
+                    const bool arr = (amask >> jj) & 1u, leaver = (lmask >> jj) & 1u;
+                    const unsigned long long bq = __ballot(arr);
+                    if (arr) Q[qn + ballot_rank(bq)] = pst + jj;
+                    qn += __popcll(bq);
+                    const unsigned long long bk = __ballot(leaver);
+                    if (leaver) {
+                        const float px = rx[jj], py = ry[jj], pz = rz[jj];
+                        const float vx = rvx[jj], vy = rvy[jj], vz = rvz[jj];
+                        const int slot = kn + ballot_rank(bk);
+                        if (slot < K0CAP) {
+                            float* e = K0 + slot * 8;
+                            e[0] = px; e[1] = py; e[2] = pz; e[3] = vx; e[4] = vy; e[5] = vz;
+                            e[6] = __int_as_float(c);
+                        } else {
+                            // buffer full: handle in place
+                            const int dex = dcell(px), dey = dcell(py), dez = dcell(pz);
+                            float wt[8];
+                            tri_weights(px - (float)dex, py - (float)dey, pz - (float)dez, wt);
+                            deposit_direct(A, G, sJ, x0, y0, z0, lx + dex, ly + dey, lz + dez, wt, qs * vx,
+                                           qs * vy, qs * vz);
+                            count_add(g, G, s_cnt, S.cnt_acc, x0, y0, z0, ax + dex, ay + dey, az + dez);
+                        }
+                    }
+                    kn += __popcll(bk);
+                }
             }
             if (MODE != MODE_REBIN && run_in > 0) {
                 if (!(jmax * (float)run_in <= g.fix_lim)) atomicOr(A.err, ERR_JSAT);
