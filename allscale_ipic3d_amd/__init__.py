@@ -64,6 +64,7 @@ class StepStats(C.Structure):
         ("max_audit_error_increment", C.c_double),
         ("max_divb_residual", C.c_double),
         ("max_energy_residual", C.c_double),
+        ("max_audit_error_evolving", C.c_double),
     ]
 
     def as_dict(self):

@@ -394,8 +394,11 @@ def evaluate(net, snaps, u_all, idx, cell_width=(1.0, 1.0, 1.0), B0=(0.0, 0.2, 1
         diff2 = np.sum((e_op - e_lf) ** 2) + np.sum((b_op - b_lf) ** 2)
         tot2 = np.sum((e_lf + np.asarray(E0)) ** 2) + np.sum((b_lf + np.asarray(B0)) ** 2)
         inc2 = np.sum(d["tE"].astype(np.float64) ** 2) + np.sum(d["tB"].astype(np.float64) ** 2)
+        evo2 = np.sum(e_lf ** 2) + np.sum(b_lf ** 2)  # the evolving field: no uniform background
         res.append({"rel_total": np.sqrt(diff2 / tot2), "rel_increment": np.sqrt(diff2 / max(inc2, 1e-300)),
-                    "persistence_rel_total": np.sqrt(inc2 / tot2)})
+                    "rel_evolving": np.sqrt(diff2 / max(evo2, 1e-300)),
+                    "persistence_rel_total": np.sqrt(inc2 / tot2),
+                    "persistence_rel_evolving": np.sqrt(inc2 / max(evo2, 1e-300))})
     psi.free(); E.free(); B.free()
     return {k: float(np.mean([r[k] for r in res])) for k in res[0]} | \
-        {"max_" + k: float(np.max([r[k] for r in res])) for k in ("rel_total", "rel_increment")}
+        {"max_" + k: float(np.max([r[k] for r in res])) for k in ("rel_total", "rel_increment", "rel_evolving")}

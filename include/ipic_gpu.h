@@ -100,6 +100,8 @@ typedef struct ipic_step_stats {
     double max_audit_error_increment; /* the same error relative to |F_lf - F_before| */
     double max_divb_residual;     /* largest max |div+ (Bc_after - Bc_before)| of an operator step */
     double max_energy_residual;   /* largest field-energy residual of an operator step */
+    double max_audit_error_evolving; /* the audited error relative to the evolving field |F_lf|
+                                     without the uniform background (dE, dBc): the audit's gate */
 } ipic_step_stats;
 
 /* FP64 conservation sums, reduced on the GPU (cell.h:1315-1355, field.h:443-459). */
@@ -186,8 +188,8 @@ int ipic_gpu_particle_step(ipic_ctx* ctx, ipic_step_stats* stats);
  * where W = 1/2 sum (|E0 + dE|^2 + c^2 |dBc|^2) vol is the energy of the field without its
  * uniform magnetic background.  Every audit_every-th operator step is also advanced by
  * leapfrog from the same state; the operator's result is kept when its error relative to the
- * leapfrog field (|F_op - F_lf| / |F_lf|, total field) is at most audit_tol, else the leapfrog
- * result is.  Needs a uniform
+ * evolving leapfrog field (|F_op - F_lf| / |F_lf - F0|, F0 the uniform background) is at most
+ * audit_tol, else the leapfrog result is.  Needs a uniform
  * background field (upload_fields with spatially constant E, Bc), nx % 4 == 0, ny % 4 == 0,
  * nz % 16 == 0 and substeps == field_substeps.  op == NULL switches the operator off. */
 typedef struct ipic_field_operator {

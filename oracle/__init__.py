@@ -102,6 +102,7 @@ def lib():
         L.or_init_species.argtypes = [P, I32, U64, P, C.c_double, C.c_double, P, P]
         L.or_ref_init.argtypes = [P, C.POINTER(RefDist), P, P]
         L.or_ref_normal_draws.argtypes = [U64, I64, P]
+        L.or_ref_init_lattice.argtypes = [P, P, C.c_double, C.c_double, P, P, P]
         L.or_ref_uniform_draws.argtypes = [C.c_uint32, C.c_double, C.c_double, I64, P]
         _lib = L
     return _lib
@@ -141,6 +142,17 @@ def ref_init(universe, pos_min, pos_max, vel_mean, vel_stddev, num_particles, q,
     out = np.zeros(int(counts.sum()), dtype=PARTICLE_DTYPE)
     lib().or_ref_init(C.byref(universe.props), C.byref(d), _p(counts), _p(out))
     return counts, out
+
+
+def ref_init_lattice(universe, ppc, qom, uth, u0, rho=1.0):
+    """The parameter-file initCells (cell.h:699-776): lattice positions and Box-Muller
+    Maxwellian velocities from the reference's per-cell std::minstd_rand streams
+    (PARTICLE_DTYPE, row-major cells)."""
+    p = i64(ppc)
+    out = np.zeros(universe.ncell * int(np.prod(p)), dtype=PARTICLE_DTYPE)
+    lib().or_ref_init_lattice(C.byref(universe.props), _p(p), float(qom), float(rho), _p(f64(uth)), _p(f64(u0)),
+                              _p(out))
+    return out
 
 
 def ref_normal_draws(seed, n):

@@ -149,6 +149,11 @@ typedef struct {
 } or_refdist;
 int or_ref_init(const or_props* P, const or_refdist* D, int64_t* counts, or_particle* out);
 void or_ref_normal_draws(uint64_t seed, int64_t n, float* out);
+/* ref_init.cpp: the parameter-file initCells (cell.h:699-776): per cell a ppc[0] x ppc[1] x ppc[2]
+ * lattice with Box-Muller Maxwellian velocities (std::minstd_rand seeded by the cell position),
+ * q = q_factor of cell.h:711-716.  out: n^3 * ppc particles, row-major cells. */
+void or_ref_init_lattice(const or_props* P, const int64_t ppc[3], double qom, double rho, const double uth[3],
+                         const double u0[3], or_particle* out);
 void or_ref_uniform_draws(uint32_t seed, double a, double b, int64_t n, double* out);
 
 #ifdef __cplusplus

@@ -54,7 +54,7 @@ def test_desc_layout_matches_header():
     # ipic_desc: 4 int32, 3+3 doubles, 2 doubles, 8 species x 2 doubles, 2 int32, 3 doubles,
     # 1 double, 3 int32, 1 uint32, 1 double
     assert ctypes.sizeof(G.Desc) == 16 + 48 + 16 + 128 + 8 + 24 + 8 + 16 + 8
-    assert ctypes.sizeof(G.StepStats) == 16 * 8
+    assert ctypes.sizeof(G.StepStats) == 17 * 8
     # ipic_field_operator: pointer, 5 floats, 2 int32, 3 doubles (4 bytes of padding)
     assert ctypes.sizeof(G.FieldOperatorDesc) == 8 + 20 + 8 + 4 + 24
     assert ctypes.sizeof(G.Conservation) == 9 * 8
