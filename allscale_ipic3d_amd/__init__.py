@@ -118,6 +118,14 @@ class RefDistribution(C.Structure):
     ]
 
 
+class RefLattice(C.Structure):
+    """ipic_ref_lattice: the parameter-file initCells (cell.h:699-776)."""
+    _fields_ = [
+        ("ppc", C.c_int32 * 3), ("nthreads", C.c_int32),
+        ("vel_thermal", C.c_double * 3), ("vel_drift", C.c_double * 3),
+    ]
+
+
 class FieldInit(C.Structure):
     """ipic_field_init: what initFields / getDipoleFieldAt (field.h:43-118) read."""
     _fields_ = [
@@ -165,7 +173,7 @@ ABI_SYMBOLS = (
     "ipic_gpu_conservation", "ipic_gpu_device_info", "ipic_gpu_field_step", "ipic_gpu_particle_step",
     "ipic_gpu_set_field_operator", "ipic_gpu_field_state", "ipic_ref_cell_counts", "ipic_ref_particles",
     "ipic_ref_last_error", "ipic_gpu_init_reference", "ipic_field_init_host", "ipic_field_init_last_error",
-    "ipic_gpu_init_fields",
+    "ipic_gpu_init_fields", "ipic_ref_lattice_particles", "ipic_ref_lattice_charge", "ipic_gpu_init_lattice",
 )
 
 _lib = None
@@ -211,9 +219,14 @@ def load_library(path: str = LIB_PATH):
     L.ipic_field_init_last_error.restype = C.c_char_p
     L.ipic_field_init_last_error.argtypes = []
     L.ipic_gpu_init_fields.argtypes = [P, C.POINTER(FieldInit), P]
+    L.ipic_ref_lattice_particles.argtypes = [C.POINTER(Desc), C.POINTER(RefLattice), P, P]
+    L.ipic_gpu_init_lattice.argtypes = [P, I32, C.POINTER(RefLattice)]
     for name in ABI_SYMBOLS:
-        if name not in ("ipic_gpu_last_error", "ipic_ref_last_error", "ipic_field_init_last_error"):
+        if name not in ("ipic_gpu_last_error", "ipic_ref_last_error", "ipic_field_init_last_error",
+                        "ipic_ref_lattice_charge"):
             getattr(L, name).restype = C.c_int
+    L.ipic_ref_lattice_charge.restype = C.c_double
+    L.ipic_ref_lattice_charge.argtypes = [P, P, C.c_double, C.c_double]
     _lib = L
     return L
 
@@ -246,6 +259,35 @@ def reference_init(desc, dist, particles=True):
     if L.ipic_ref_particles(C.byref(desc), C.byref(dist), _ptr(counts), _ptr(pos), _ptr(vel)) != 0:
         raise IpicError(L.ipic_ref_last_error().decode())
     return counts, pos, vel
+
+
+def ref_lattice(ppc, vel_thermal, vel_drift=(0.0, 0.0, 0.0), nthreads=0):
+    lat = RefLattice()
+    lat.ppc[:] = [int(v) for v in ppc]
+    lat.nthreads = int(nthreads)
+    lat.vel_thermal[:] = [float(v) for v in vel_thermal]
+    lat.vel_drift[:] = [float(v) for v in vel_drift]
+    return lat
+
+
+def lattice_particles(desc, lat):
+    """The parameter-file initCells on the host (ipic_ref_lattice_particles): positions and
+    velocities (n x 3 doubles, row-major cells, lattice order within a cell).  No GPU is used."""
+    L = load_library()
+    ncell = int(desc.n[0]) * int(desc.n[1]) * int(desc.n[2])
+    n = ncell * int(lat.ppc[0]) * int(lat.ppc[1]) * int(lat.ppc[2])
+    pos = np.zeros((n, 3), dtype=np.float64)
+    vel = np.zeros((n, 3), dtype=np.float64)
+    if L.ipic_ref_lattice_particles(C.byref(desc), C.byref(lat), _ptr(pos), _ptr(vel)) != 0:
+        raise IpicError(L.ipic_ref_last_error().decode())
+    return pos, vel
+
+
+def lattice_charge(cell_width, ppc, qom, rho=1.0):
+    """q of every particle of the parameter-file initCells (cell.h:711-716)."""
+    L = load_library()
+    return float(L.ipic_ref_lattice_charge(_ptr(_f64(cell_width)), _ptr(np.asarray(ppc, dtype=np.int32)),
+                                           float(qom), float(rho)))
 
 
 def field_init_host(desc, init):
@@ -369,6 +411,11 @@ class Simulation:
         counts = np.zeros(self.ncell, dtype=np.int64)
         self._check(self._L.ipic_gpu_init_reference(self._h, int(species), C.byref(dist), _ptr(counts)))
         return counts
+
+    def init_lattice(self, species, lat):
+        """The parameter-file initCells (cell.h:699-776) into `species` (ipic_gpu_init_lattice); the
+        species' q should be lattice_charge(...) of the same lattice."""
+        self._check(self._L.ipic_gpu_init_lattice(self._h, int(species), C.byref(lat)))
 
     def init_fields(self, init):
         """initFields + initBcFields (field.h:81-138) as the field state (ipic_gpu_init_fields);

@@ -2632,6 +2632,26 @@ int ipic_gpu_init_reference(ipic_ctx* ctx, int32_t species, const ipic_ref_distr
     });
 }
 
+int ipic_gpu_init_lattice(ipic_ctx* ctx, int32_t species, const ipic_ref_lattice* lat) {
+    return guarded(ctx, [&]() -> int {
+        if (int rc = check_ctx(ctx)) return rc;
+        if (species < 0 || species >= ctx->d.nspecies) return fail(ctx, IPIC_E_ARG, "species out of range");
+        if (!lat) return fail(ctx, IPIC_E_ARG, "null lattice description");
+        const int64_t nc = (int64_t)ctx->d.n[0] * ctx->d.n[1] * ctx->d.n[2];
+        int64_t per = 1;
+        for (int k = 0; k < 3; ++k) {
+            if (lat->ppc[k] <= 0) return fail(ctx, IPIC_E_ARG, "particles per cell must be positive on every axis");
+            per *= lat->ppc[k];
+        }
+        if (per >= (1LL << 31) / nc) return fail(ctx, IPIC_E_RANGE, "at most 2^31-1 particles per species");
+        const int64_t total = nc * per;
+        std::vector<double> pos((size_t)total * 3), vel((size_t)total * 3);
+        if (int rc = ipic_ref_lattice_particles(&ctx->d, lat, pos.data(), vel.data()))
+            return fail(ctx, rc, ipic_ref_last_error());
+        return ipic_gpu_upload_particles(ctx, species, total, pos.data(), vel.data());
+    });
+}
+
 int ipic_gpu_init_particles(ipic_ctx* ctx, int32_t species, const int32_t* ppc, int32_t uniform_ppc,
                             const double vth[3], const double drift[3], uint64_t seed) {
     return guarded(ctx, [&]() -> int {

@@ -333,7 +333,75 @@ static int particles_impl(const ipic_desc* desc, const ipic_ref_distribution* di
     return 0;
 }
 
+// Parameter-file initCells (cell.h:699-776).  Every cell draws from its own minstd stream, so
+// cells run on any thread in any order; the k-th particle of cell c lands at c * per + k.
+static int lattice_impl(const ipic_desc* desc, const ipic_ref_lattice* lat, double* pos, double* vel) {
+    Geometry g;
+    if (int rc = geometry(desc, g)) return rc;
+    if (!lat) return ref_fail(IPIC_E_ARG, "null lattice description");
+    if (!pos || !vel) return ref_fail(IPIC_E_ARG, "null output arrays");
+    for (int k = 0; k < 3; ++k) {
+        if (lat->ppc[k] <= 0) return ref_fail(IPIC_E_ARG, "particles per cell must be positive on every axis");
+        if (!std::isfinite(lat->vel_thermal[k]) || !std::isfinite(lat->vel_drift[k]))
+            return ref_fail(IPIC_E_ARG, "non-finite thermal or drift velocity");
+    }
+    const int64_t per = (int64_t)lat->ppc[0] * lat->ppc[1] * lat->ppc[2];
+    double sub[3];
+    for (int k = 0; k < 3; ++k) sub[k] = g.w[k] / (double)lat->ppc[k];
+    const int T = thread_count(lat->nthreads, g.ncell);
+    std::atomic<int64_t> next{0};
+    std::vector<std::thread> pool;
+    for (int t = 0; t < T; ++t) {
+        pool.emplace_back([&]() {
+            for (;;) {
+                const int64_t c0 = next.fetch_add(64);
+                if (c0 >= g.ncell) return;
+                for (int64_t c = c0; c < std::min<int64_t>(g.ncell, c0 + 64); ++c) {
+                    const int64_t cx = c / (g.n[1] * g.n[2]), cy = (c / g.n[2]) % g.n[1], cz = c % g.n[2];
+                    const double corner[3] = {g.origin[0] + (double)cx * g.w[0], g.origin[1] + (double)cy * g.w[1],
+                                              g.origin[2] + (double)cz * g.w[2]};
+                    // the seed is the cell's position folded as x * 10000 + y * 100 + z (unsigned)
+                    Lcg s((uint32_t)(cx * 10000 + cy * 100 + cz));
+                    auto unit = [&s]() { return (double)s.next() / (double)(kM - 1); };
+                    int64_t i = c * per;
+                    for (int32_t a = 0; a < lat->ppc[0]; ++a)
+                        for (int32_t b = 0; b < lat->ppc[1]; ++b)
+                            for (int32_t d = 0; d < lat->ppc[2]; ++d, ++i) {
+                                const int32_t l[3] = {a, b, d};
+                                for (int k = 0; k < 3; ++k) pos[3 * i + k] = ((double)l[k] + 0.5) * sub[k] + corner[k];
+                                // Box-Muller: two radii and two angles per particle, drawn r, phi, r, phi
+                                const double r0 = std::sqrt(-2.0 * std::log(1.0 - 0.999999 * unit()));
+                                const double p0 = 2.0 * M_PI * unit();
+                                const double r1 = std::sqrt(-2.0 * std::log(1.0 - 0.999999 * unit()));
+                                const double p1 = 2.0 * M_PI * unit();
+                                vel[3 * i + 0] = lat->vel_drift[0] + lat->vel_thermal[0] * (r0 * std::cos(p0));
+                                vel[3 * i + 1] = lat->vel_drift[1] + lat->vel_thermal[1] * (r0 * std::sin(p0));
+                                vel[3 * i + 2] = lat->vel_drift[2] + lat->vel_thermal[2] * (r1 * std::cos(p1));
+                            }
+                }
+            }
+        });
+    }
+    for (auto& p : pool) p.join();
+    return 0;
+}
+
 // no C++ exception crosses the C ABI: a failed allocation or thread start is an error code
+int ipic_ref_lattice_particles(const ipic_desc* desc, const ipic_ref_lattice* lat, double* pos, double* vel) {
+    try {
+        return lattice_impl(desc, lat, pos, vel);
+    } catch (const std::exception& e) {
+        return ref_fail(IPIC_E_RANGE, std::string("reference init: ") + e.what());
+    }
+}
+
+double ipic_ref_lattice_charge(const double cell_width[3], const int32_t ppc[3], double qom, double rho) {
+    // q_factor of cell.h:711-716, the same operations in the same order
+    const double sign = qom / std::fabs(qom);
+    const double vol = cell_width[0] * cell_width[1] * cell_width[2];
+    return sign * vol / (double)((int64_t)ppc[0] * ppc[1] * ppc[2]) * rho / (16.0 * std::atan(1.0));
+}
+
 int ipic_ref_cell_counts(const ipic_desc* desc, const ipic_ref_distribution* dist, int64_t* counts) {
     try {
         return cell_counts_impl(desc, dist, counts);

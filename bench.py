@@ -44,10 +44,21 @@ def algorithmic_bytes_per_particle(ppc: float) -> float:
 PUSH_SOURCE = os.path.join(ROOT, "allscale_ipic3d_amd", "csrc", "ipic_gpu.hip")
 
 
-def source_sha256() -> str:
+PUSH_SECTION_END = b"// field kernels (N-periodic, row-major float4)"
+
+
+def source_sha256(text: bytes = None) -> str:
+    """Hash of the push kernel's device code: csrc/ipic_gpu.hip from its start up to the field
+    kernels' banner (constants, helpers, k_particles).  Host-side edits further down leave it as
+    is; they are not covered, the launch set-up included."""
     import hashlib
-    with open(PUSH_SOURCE, "rb") as f:
-        return hashlib.sha256(f.read()).hexdigest()
+    if text is None:
+        with open(PUSH_SOURCE, "rb") as f:
+            text = f.read()
+    end = text.find(PUSH_SECTION_END)
+    if end < 0:
+        raise RuntimeError("push kernel section banner not found in " + PUSH_SOURCE)
+    return hashlib.sha256(text[:end]).hexdigest()
 
 
 def load_traffic(cells: int, ppc: int):
@@ -64,7 +75,7 @@ def load_traffic(cells: int, ppc: int):
     except (OSError, ValueError):
         return None, "unreadable record"
     if d.get("source_sha256") != source_sha256():
-        return None, "stale: measured on another version of csrc/ipic_gpu.hip"
+        return None, "stale: measured on another version of the push kernel (csrc/ipic_gpu.hip)"
     return float(d["hbm_bytes_per_launch"]), "measured on this source"
 
 

@@ -10,6 +10,9 @@
 //   distribution::uniform_pos_normal_speed, createUniverseFromDistribution, Universe::initCells
 //                               cell.h:405-440, universe.h:85-103, cell.h:512-654 (the
 //                               reference's own random streams)
+//   InitProperties, createUniverseFromParams, Universe::initCells(InitProperties, ...),
+//   Universe::initFields        init_properties.h:14-45, universe.h:53-75, cell.h:699-776,
+//                               field.h:43-138 (the parameter-file set-up)
 //
 // The reference aborts on a failed assert_*; here every failed C call throws
 // ipic3d_gpu::Error carrying the IPIC_E_* code and the library's message.  Header-only,
@@ -82,6 +85,16 @@ struct uniform_pos_normal_speed {
         : minPos(minPos_), maxPos(maxPos_), center(center_), stddev(stddev_), seed(seed_) {}
 };
 }  // namespace distribution
+
+// init_properties.h:14-45: the properties that only initialisation reads, one entry per species
+// in particlesPerCell / driftVelocity (the set-up uses the first)
+struct InitProperties {
+    std::uint64_t numSteps = 1;
+    std::vector<std::array<unsigned, 3>> particlesPerCell;
+    std::vector<Vector3> driftVelocity;
+    Vector3 magneticField{0.0, 0.0, 0.0};
+    double rhoInit = 1.0;
+};
 
 // simulator.h:37-40
 struct DurationMeasurement {
@@ -213,6 +226,39 @@ class Universe {
         return counts;
     }
 
+    // the parameter-file initCells (cell.h:699-776): init.particlesPerCell[0] particles on a
+    // lattice in every cell, velocities driftVelocity[0] + thermalVelocity * the Box-Muller
+    // Maxwellian of each cell's own std::minstd_rand stream, into `species`; the species' q should
+    // be latticeCharge(...) of the same lattice (the charge every such particle carries)
+    void initCells(const InitProperties& init, const Vector3& thermalVelocity, int species = 0) {
+        if (init.particlesPerCell.empty() || init.driftVelocity.empty())
+            throw Error(IPIC_E_ARG, "InitProperties needs particlesPerCell and driftVelocity of the first species");
+        ipic_ref_lattice lat{};
+        for (int k = 0; k < 3; ++k) {
+            lat.ppc[k] = (int32_t)init.particlesPerCell[0][k];
+            lat.vel_thermal[k] = thermalVelocity[k];
+            lat.vel_drift[k] = init.driftVelocity[0][k];
+        }
+        check(ctx_, ipic_gpu_init_lattice(ctx_, (int32_t)species, &lat));
+    }
+
+    // initFields + initBcFields (field.h:81-138) of the dipole use case: E = -drift x B0, B = B0 plus
+    // the dipole of externalMagneticField.z and planetRadius at objectCenter; returns the field
+    // energies writeOutputData logs for them (E, B + Bext)
+    std::pair<double, double> initFields(const InitProperties& init) {
+        ipic_field_init f{};
+        for (int k = 0; k < 3; ++k) {
+            f.drift[k] = init.driftVelocity.empty() ? 0.0 : init.driftVelocity[0][k];
+            f.B0[k] = init.magneticField[k];
+            f.object_center[k] = props_.objectCenter[k];
+        }
+        f.planet_radius = props_.planetRadius;
+        f.external_Bz = props_.externalMagneticField[2];
+        double en[2] = {0.0, 0.0};
+        check(ctx_, ipic_gpu_init_fields(ctx_, &f, en));
+        return {en[0], en[1]};
+    }
+
     // countParticlesInDomain, cell.h:87-100
     std::int64_t countParticlesInDomain() const {
         int64_t n = 0;
@@ -276,6 +322,26 @@ inline Universe createUniverseFromDistribution(const UniverseProperties& props, 
                                                const distribution::uniform_pos_normal_speed& dist) {
     Universe u(props, {{kElementaryCharge, kElementaryCharge / kProtonMass}});
     u.initCells(numParticles, dist);
+    return u;
+}
+
+// q_factor of cell.h:711-716: the charge of every particle of the parameter-file initCells
+inline double latticeCharge(const UniverseProperties& props, const std::array<unsigned, 3>& particlesPerCell,
+                            double qom, double rhoInit) {
+    const double w[3] = {props.cellWidth[0], props.cellWidth[1], props.cellWidth[2]};
+    const int32_t ppc[3] = {(int32_t)particlesPerCell[0], (int32_t)particlesPerCell[1], (int32_t)particlesPerCell[2]};
+    return ipic_ref_lattice_charge(w, ppc, qom, rhoInit);
+}
+
+// createUniverseFromParams (universe.h:53-75) for what it reads of Parameters besides
+// InitProperties: the first species' qom and thermal velocity (uth, vth, wth).  One species of
+// (latticeCharge, qom); cells from the parameter-file initCells, fields from initFields.
+inline Universe createUniverseFromParams(const UniverseProperties& props, const InitProperties& init, double qom,
+                                         const Vector3& thermalVelocity) {
+    if (init.particlesPerCell.empty()) throw Error(IPIC_E_ARG, "InitProperties needs particlesPerCell");
+    Universe u(props, {{latticeCharge(props, init.particlesPerCell[0], qom, init.rhoInit), qom}});
+    u.initCells(init, thermalVelocity);
+    u.initFields(init);
     return u;
 }
 

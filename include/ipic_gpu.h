@@ -248,6 +248,31 @@ const char* ipic_ref_last_error(void);
  * receives the phase-1 counts. */
 int ipic_gpu_init_reference(ipic_ctx* ctx, int32_t species, const ipic_ref_distribution* dist, int64_t* counts);
 
+/* The parameter-file form of initCells (cell.h:699-776), which createUniverseFromParams
+ * (universe.h:53-75) calls: every cell holds ppc[0] x ppc[1] x ppc[2] particles at the centres
+ * of a regular sub-lattice, position (l + 0.5) * cell_width / ppc + cell corner per axis; each
+ * cell (x, y, z) draws from its own std::minstd_rand seeded with (unsigned)(10000 x + 100 y + z),
+ * four draws u / max() per particle, and sets the Box-Muller Maxwellian
+ *   v = vel_drift + vel_thermal * (r0 cos t0, r0 sin t0, r1 cos t1),
+ *   r = sqrt(-2 ln(1 - 0.999999 u)), t = 2 pi u      (draw order r0, t0, r1, t1).
+ * The members are the Parameters keys the reference reads: npcel{x,y,z}, {u,v,w}th, {u,v,w}0 of
+ * the first species.  Host computation, bit-identical to the reference's streams. */
+typedef struct ipic_ref_lattice {
+    int32_t ppc[3];               /* particles per cell along each axis */
+    int32_t nthreads;             /* host threads, 0: all */
+    double vel_thermal[3];        /* uth, vth, wth */
+    double vel_drift[3];          /* u0, v0, w0 */
+} ipic_ref_lattice;
+
+/* The particles of all cells: ncell * ppc[0] * ppc[1] * ppc[2] of them, row-major cells, each
+ * cell's in lattice order (x slowest); pos, vel: that many x 3 doubles (absolute positions). */
+int ipic_ref_lattice_particles(const ipic_desc* desc, const ipic_ref_lattice* lat, double* pos, double* vel);
+/* The charge every particle of that initialisation carries (q_factor, cell.h:711-716):
+ * sign(qom) * cell volume / (ppc product) * rho / (4 pi).  Set it as the species' q. */
+double ipic_ref_lattice_charge(const double cell_width[3], const int32_t ppc[3], double qom, double rho);
+/* ipic_ref_lattice_particles, then ipic_gpu_upload_particles of the result into `species`. */
+int ipic_gpu_init_lattice(ipic_ctx* ctx, int32_t species, const ipic_ref_lattice* lat);
+
 /* The initial field of the reference's dipole use case (UseCase::Dipole):
  *   getDipoleFieldAt (field.h:43-78): E = -drift x B0; B = B0 + Bext with
  *     Bext = -external_Bz a^3 (3 x z, 3 y z, 2 z^2 - x^2 - y^2) / r^5 for r > a (r = location -

@@ -9,12 +9,17 @@
 //                                                 (src/ipic3d.cxx:41-99): createUniverseFrom-
 //                                                 Distribution, 30 steps of the loop as shipped,
 //                                                 <prefix>.out in the reference's format
+//   ipic3d_mirror_test --params-case <n> <ppc>      createUniverseFromParams of a lattice
+//                                                 set-up (qom -25, uth 0.045, u0 0.02,
+//                                                 B0 = (0, 0, 1e-4), cells of 10 / n): prints
+//                                                 the conserved quantities of cycle 0
 //
 // input.bin (little-endian): int64 n, int64 steps, int64 nparticles, double dt, then per
 // particle 8 doubles (position, velocity, q, qom), then n^3 x 3 doubles E and n^3 x 3 Bc.
 // output.bin: int64 particle count, n^3 int64 cell counts, double kinetic energy, double
 // firstStep, double remainingSteps, int64 1 if the operator-policy guard threw.
 #include <cstdio>
+#include <cstdlib>
 #include <fstream>
 #include <iostream>
 #include <set>
@@ -67,6 +72,31 @@ int main(int argc, char** argv) {
             outputNumberOfParticlesPerCell(u, std::string(argv[2]) + ".out");
             std::printf("{\"particles\": %lld, \"first\": %g, \"remaining\": %g}\n",
                         (long long)u.countParticlesInDomain(), d.firstStep, d.remainingSteps);
+        } catch (const Error& e) {
+            std::fprintf(stderr, "error %d: %s\n", e.code(), e.what());
+            return 1;
+        }
+        return 0;
+    }
+    if (argc == 4 && std::string(argv[1]) == "--params-case") {
+        const int n = std::atoi(argv[2]);
+        const unsigned ppc = (unsigned)std::atoi(argv[3]);
+        UniverseProperties p;
+        p.size = {n, n, n};
+        p.cellWidth = {10.0 / n, 10.0 / n, 10.0 / n};
+        p.dt = 0.15;
+        p.fieldSubsteps = 0;
+        InitProperties init;
+        init.particlesPerCell = {{ppc, ppc, ppc}};
+        init.driftVelocity = {{0.02, 0.0, 0.0}};
+        init.magneticField = {0.0, 0.0, 1e-4};
+        try {
+            Universe u = createUniverseFromParams(p, init, -25.0, {0.045, 0.045, 0.045});
+            const std::pair<double, double> en = u.initFields(init);
+            const ipic_conservation c = u.conservation();
+            std::printf("{\"particles\": %lld, \"momentum\": %.17g, \"kinetic_energy\": %.17g, "
+                        "\"E_energy\": %.17g, \"B_energy\": %.17g}\n",
+                        (long long)u.countParticlesInDomain(), c.momentum, c.kinetic_energy, en.first, en.second);
         } catch (const Error& e) {
             std::fprintf(stderr, "error %d: %s\n", e.code(), e.what());
             return 1;

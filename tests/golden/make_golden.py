@@ -14,6 +14,11 @@ Outputs (committed fixtures, data only):
   test_conserved.txt      -- outputs/testConservedQuantities.out: per-cycle momentum, E and B
   tiny_conserved.txt         field energies and kinetic energy of those runs (simulator.h:73-91);
                              tiny_conserved.txt from outputs/tinyConservedQuantities.out
+  test_conserved_static.txt  -- outputs/testConservedQuantities.static.out and
+  tiny_conserved_static.txt     tinyConservedQuantities.static.out: the same columns for runs whose
+                             particles come from the parameter-file initCells (cell.h:699-776,
+                             5^3 and 8^3 particles per cell, qom -25, uth 0.045, u0 0.02) in the
+                             constant field of B0 and the drift (row 0 is the initial state)
   test_case.inp           -- the key = value lines of inputs/test.inp / tiny.inp that the
   tiny_case.inp              driver reads (comments and unused keys dropped)
 Row format of the two trajectory files (as in the reference): step t x y z vx vy vz.
@@ -37,7 +42,9 @@ def main():
     subset("results/exb_drift.txt", "exb_drift_rows.txt", 100)
     for src, dst in (("outputs/test.out", "test_out_counts.txt"), ("outputs/tiny.out", "tiny_out_counts.txt"),
                      ("outputs/testConservedQuantities.out", "test_conserved.txt"),
-                     ("outputs/tinyConservedQuantities.out", "tiny_conserved.txt")):
+                     ("outputs/tinyConservedQuantities.out", "tiny_conserved.txt"),
+                     ("outputs/testConservedQuantities.static.out", "test_conserved_static.txt"),
+                     ("outputs/tinyConservedQuantities.static.out", "tiny_conserved_static.txt")):
         with open(os.path.join(REF, src)) as f:
             data = f.read()
         with open(os.path.join(HERE, dst), "w") as f:

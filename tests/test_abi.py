@@ -14,7 +14,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def _declared(header):
     txt = open(os.path.join(ROOT, "include", header)).read()
-    return sorted(set(re.findall(r"^(?:int|int64_t|const char\*)\s+(ipic_\w+)\s*\(", txt, flags=re.M)))
+    return sorted(set(re.findall(r"^(?:int|int64_t|double|const char\*)\s+(ipic_\w+)\s*\(", txt, flags=re.M)))
 
 
 def test_header_declares_binding_symbols():

@@ -112,3 +112,24 @@ def test_cpp_mirror_driver_case(tmp_path):
     assert n == (8, 8, 8)
     res = D.run(os.path.join(ROOT, "tests", "golden", "test_case.inp"), write=False)
     np.testing.assert_array_equal(cells, res["counts"])
+
+
+@pytest.mark.gpu
+@need_exe
+@pytest.mark.parametrize("case", ["test", "tiny"])
+def test_cpp_mirror_params_case(case):
+    """createUniverseFromParams (parameter-file initCells + initFields) through the C++ mirror
+    gives row 0 of outputs/{test,tiny}ConservedQuantities.static.out: 8^3 cells x 5^3 and 16^3 x
+    8^3 lattice particles."""
+    import json
+    n, ppc, gold = {"test": (8, 5, "test_conserved_static.txt"), "tiny": (16, 8, "tiny_conserved_static.txt")}[case]
+    r = subprocess.run([EXE, "--params-case", str(n), str(ppc)], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    got = json.loads(r.stdout.strip().splitlines()[-1])
+    with open(os.path.join(ROOT, "tests", "golden", gold)) as f:
+        mom, e_en, b_en, ke = (float(v) for v in f.read().splitlines()[1].split()[1:])
+    assert got["particles"] == n ** 3 * ppc ** 3
+    assert abs(got["momentum"] / mom - 1) < 1e-5 and abs(got["kinetic_energy"] / ke - 1) < 1e-5
+    assert abs(got["E_energy"] / e_en - 1) < 5e-6
+    assert got["B_energy"] <= b_en and abs(got["B_energy"] / b_en - 1) < 1e-3

@@ -105,3 +105,36 @@ def test_tiny_case_matches_oracle_and_reference():
     assert np.abs(final - gold).sum() / 2 <= 0.01 * gold.sum()
     sim.close()
     u.close()
+
+
+@pytest.mark.parametrize("case", ["test", "tiny"])
+def test_lattice_init_on_device_matches_the_logged_initial_state(case):
+    """The parameter-file initCells (cell.h:699-776) uploaded by ipic_gpu_init_lattice: the
+    device holds the host lattice particle for particle, and its conservation sums and the field
+    energies of ipic_gpu_init_fields give row 0 of outputs/{test,tiny}ConservedQuantities.static.out
+    (8^3 x 5^3 and 16^3 x 8^3 particles, qom -25, uth 0.045, u0 0.02, B0 = (0, 0, 1e-4))."""
+    n, w, ppc, gold = {"test": (8, 1.25, 5, "test_conserved_static.txt"),
+                       "tiny": (16, 0.625, 8, "tiny_conserved_static.txt")}[case]
+    with open(os.path.join(GOLD, gold)) as f:
+        mom_log, e_log, b_log, ke_log = (float(v) for v in f.read().splitlines()[1].split()[1:])
+    q = A.lattice_charge((w,) * 3, (ppc,) * 3, -25.0)
+    sim = A.Simulation((n,) * 3, cell_width=(w,) * 3, origin=(0.0, 0.0, 0.0), dt=0.15, c=1.0,
+                       species=((q, -25.0),), field_substeps=0, flags=A.FLAG_REFLECT)
+    lat = A.ref_lattice((ppc,) * 3, (0.045,) * 3, (0.02, 0.0, 0.0))
+    sim.init_lattice(0, lat)
+    e_en, b_en = sim.init_fields(A.field_init(drift=(0.02, 0.0, 0.0), B0=(0.0, 0.0, 1e-4)))
+    d = A.Desc()
+    d.n[:] = [n] * 3
+    d.cell_width[:] = [w] * 3
+    d.nspecies = 1
+    pos, vel = A.lattice_particles(d, lat)
+    p = sim.download_particles(0)
+    np.testing.assert_array_equal(sim.cell_counts(), np.full(n ** 3, ppc ** 3))
+    np.testing.assert_allclose(p["pos"], pos, rtol=0, atol=1e-6 * w)    # FP32 offsets in the cell
+    np.testing.assert_allclose(p["vel"], vel, rtol=1e-7, atol=1e-12)
+    c = sim.conservation()
+    assert abs(c["momentum"] / mom_log - 1) < 1e-5
+    assert abs(c["kinetic_energy"] / ke_log - 1) < 1e-5
+    assert abs(e_en / e_log - 1) < 5e-6
+    assert b_en <= b_log and abs(b_en / b_log - 1) < 1e-3
+    sim.close()
