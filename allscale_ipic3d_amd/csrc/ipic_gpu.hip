@@ -512,8 +512,13 @@ __device__ __forceinline__ bool finite3(float x, float y, float z) {
 
 __device__ __forceinline__ bool alive(float x, float y, float z) { return x == x && y == y && z == z; }
 
+// 0 <= x, y, z <= 1 on the bit patterns: a non-negative float orders like its pattern as an
+// unsigned integer, and a negative one (sign bit set) or a NaN lies above the pattern of 1.0.
+// -0.0 counts as outside; offsets are never -0.0 (uploads store +0.0 for it, and the push only
+// adds to offsets, which cannot produce -0.0 from a +0.0 or non-zero start).
 __device__ __forceinline__ bool in_cell(float x, float y, float z) {
-    return x >= 0.0f && x <= 1.0f && y >= 0.0f && y <= 1.0f && z >= 0.0f && z <= 1.0f;
+    const unsigned m = max(max(__float_as_uint(x), __float_as_uint(y)), __float_as_uint(z));
+    return m <= 0x3f800000u;
 }
 
 // Particle arrays are read and written through buffer resources: one resource spans the six
@@ -2566,7 +2571,7 @@ int ipic_gpu_upload_particles(ipic_ctx* ctx, int32_t species, int64_t n, const d
                 if (cc < 0) cc = 0;
                 ci[k] = (int)cc;
                 float o = (float)(qd - (double)cc);
-                if (o < 0.0f) o = 0.0f;
+                if (!(o > 0.0f)) o = 0.0f;  // also -0.0 -> +0.0: the kernel's cell test reads bit patterns
                 if (o > 1.0f) o = 1.0f;
                 off[3 * i + k] = o;
             }
