@@ -344,14 +344,16 @@ def pack_all(net, snaps):
 
 
 def train(net, snaps, u_all, train_idx, varE, varB, epochs=1, lr=1e-3, lr_end=None, seed=0, log=None,
-          log_every=200):
+          log_every=200, k0=0, total=None):
     """Adam over the pairs of train_idx, one pair per step, shuffled each epoch; cosine decay
-    from lr to lr_end.  Returns the per-step losses."""
+    from lr to lr_end over `total` steps (default: this call's), of which this call runs steps
+    k0, k0 + 1, ... (a schedule spread over several calls on chunks of pairs).  Returns the
+    per-step losses."""
     rng = np.random.default_rng(seed)
     lr_end = lr * 0.05 if lr_end is None else lr_end
-    total = epochs * len(train_idx)
+    total = epochs * len(train_idx) if total is None else int(total)
     losses = []
-    k = 0
+    k = int(k0)
     t0 = time.perf_counter()
     for ep in range(epochs):
         order = rng.permutation(train_idx)
@@ -362,7 +364,7 @@ def train(net, snaps, u_all, train_idx, varE, varB, epochs=1, lr=1e-3, lr_end=No
             net.adam(a)
             losses.append(l)
             k += 1
-            if log and k % log_every == 0:
+            if log and (k - k0) % log_every == 0:
                 log(f"step {k}/{total} epoch {ep} loss {np.mean(losses[-log_every:]):.4e} "
                     f"({time.perf_counter() - t0:.1f} s)")
     net.sync()
