@@ -318,7 +318,6 @@ __device__ __forceinline__ void deposit_direct(const PushArgs& A, const TG& G, l
                                            __HIP_MEMORY_SCOPE_AGENT);
                     __hip_atomic_fetch_add(o + 2, fix_of(jz * ww, g.fix_scale), __ATOMIC_RELAXED,
                                            __HIP_MEMORY_SCOPE_AGENT);
-                    *A.jflag = 1;
                 }
             }
 }
@@ -970,6 +969,37 @@ __global__ __launch_bounds__(NTP) IPIC_PUSH_ATTR void k_particles(PushArgs A) {
 #pragma unroll
             for (int m = 0; m < 8; ++m) mom[0][m] = mom[1][m] = mom[2][m] = 0.0f;
             float jmax = 0.0f;  // largest |j| component of the run: bounds every corner sum / run_in
+            // this run's arrivals: particles that sit in another cell (queued by input index)
+            unsigned amask = 0;
+#pragma unroll
+            for (int jj = 0; jj < KRUN; ++jj)
+                if (!((hm >> jj) & 1u) && jj < len && alive(rx[jj], ry[jj], rz[jj])) amask |= 1u << jj;
+            // A run with a single slot that is not home (an arrival, or a particle removed in the
+            // previous step) closes the gap: the home particles move down one register, and the
+            // run is stored like a run of home particles only.  The arrival itself is reloaded by
+            // its input index when the queue is drained.
+            if (MODE != MODE_DEPOSIT) {
+                const unsigned gap = ((1u << len) - 1u) & ~hm;
+                const bool one_gap = gap != 0u && (gap & (gap - 1u)) == 0u;
+                if (__ballot(one_gap)) {
+                    const int zg = one_gap ? __builtin_ctz(gap) : KRUN;
+#pragma unroll
+                    for (int i = 0; i + 1 < KRUN; ++i) {
+                        const bool sh = i >= zg;
+                        rx[i] = sh ? rx[i + 1] : rx[i];
+                        ry[i] = sh ? ry[i + 1] : ry[i];
+                        rz[i] = sh ? rz[i + 1] : rz[i];
+                        rvx[i] = sh ? rvx[i + 1] : rvx[i];
+                        rvy[i] = sh ? rvy[i + 1] : rvy[i];
+                        rvz[i] = sh ? rvz[i + 1] : rvz[i];
+                        if (TRACK) rid[i] = sh ? rid[i + 1] : rid[i];
+                    }
+                    if (one_gap) {
+                        --len;
+                        hm = (1u << len) - 1u;
+                    }
+                }
+            }
             // a run of home particles only (the common case) takes consecutive slots: its six
             // arrays go out in a few stores covering whole lines across the wave
             const bool whole = (MODE != MODE_DEPOSIT) && len > 0 && hm == (1u << len) - 1u;
@@ -999,13 +1029,12 @@ __global__ __launch_bounds__(NTP) IPIC_PUSH_ATTR void k_particles(PushArgs A) {
                             if (jj < len) S.id_out[o0 + jj] = rid[jj];
                 }
             }
-            unsigned amask = 0, lmask = 0;  // this run's arrivals (queue entries) and leavers
+            unsigned lmask = 0;  // this run's leavers (register positions after a closed gap)
 #pragma unroll
             for (int jj = 0; jj < KRUN; ++jj) {
                 const float px = rx[jj], py = ry[jj], pz = rz[jj];
                 const float vx = rvx[jj], vy = rvy[jj], vz = rvz[jj];
                 const bool home = (hm >> jj) & 1u;
-                if (!home && alive(px, py, pz)) amask |= 1u << jj;  // sits in another cell
                 if (MODE != MODE_DEPOSIT && !whole && home) {
                     const int oidx = a_start + base + rank;
                     ++rank;
@@ -1245,6 +1274,7 @@ __global__ __launch_bounds__(256) void k_jsum(FieldArgs F) {
 }
 
 __global__ void k_clear_flag(int* f) { *f = 0; }
+__global__ void k_set_flag(int* f) { *f = 1; }
 
 // Drops overflow deposits that were never summed (a deposit pass replaces the partials).
 __global__ void k_discard_overflow(const int* flag, long long* jover, long long n) {
@@ -1983,6 +2013,10 @@ int launch_particles(ipic_ctx* ctx, int mode) {
         else if (mode == MODE_DEPOSIT) launch_opt<MODE_DEPOSIT, false>(opt, grid, block, lds, st, A);
         else launch_opt<MODE_REBIN, false>(opt, grid, block, lds, st, A);
     }
+    // A deposit pass may leave deposits in the global overflow accumulator (particles that cross
+    // a tile face): it is marked for the next J sum here, once, rather than by a store of every
+    // depositing lane to one shared word.
+    if (mode != MODE_REBIN) hipLaunchKernelGGL(k_set_flag, dim3(1), dim3(1), 0, st, ctx->jflag);
     HIPCK(ctx, hipGetLastError());
     if (mode != MODE_REBIN) ctx->j_summed = false;
     if (mode != MODE_DEPOSIT)
