@@ -977,8 +977,10 @@ __global__ __launch_bounds__(NTP) IPIC_PUSH_ATTR void k_particles(PushArgs A) {
             // A run with a single slot that is not home (an arrival, or a particle removed in the
             // previous step) closes the gap: the home particles move down one register, and the
             // run is stored like a run of home particles only.  The arrival itself is reloaded by
-            // its input index when the queue is drained.
-            if (MODE != MODE_DEPOSIT) {
+            // its input index when the queue is drained.  (Not in the instances that carry particle
+            // ids, where the extra registers would spill; they store such runs particle by
+            // particle.)
+            if (MODE != MODE_DEPOSIT && !TRACK) {
                 const unsigned gap = ((1u << len) - 1u) & ~hm;
                 const bool one_gap = gap != 0u && (gap & (gap - 1u)) == 0u;
                 if (__ballot(one_gap)) {
@@ -992,7 +994,6 @@ __global__ __launch_bounds__(NTP) IPIC_PUSH_ATTR void k_particles(PushArgs A) {
                         rvx[i] = sh ? rvx[i + 1] : rvx[i];
                         rvy[i] = sh ? rvy[i + 1] : rvy[i];
                         rvz[i] = sh ? rvz[i + 1] : rvz[i];
-                        if (TRACK) rid[i] = sh ? rid[i + 1] : rid[i];
                     }
                     if (one_gap) {
                         --len;
