@@ -404,3 +404,33 @@ def evaluate(net, snaps, u_all, idx, cell_width=(1.0, 1.0, 1.0), B0=(0.0, 0.2, 1
     psi.free(); E.free(); B.free()
     return {k: float(np.mean([r[k] for r in res])) for k in res[0]} | \
         {"max_" + k: float(np.max([r[k] for r in res])) for k in ("rel_total", "rel_increment", "rel_evolving")}
+
+
+def forward_passes(n, reps, seed=0):
+    """`reps` inference passes of an untrained operator on an n^3 grid of random fields (a fixed
+    workload for kernel-level profiling of the convolutions)."""
+    V = n ** 3
+    scales = {"in_E": 1e-3, "in_B": 1e-3, "in_J": 1e-4, "out_E": 1e-4, "out_P": 5e-5}
+    net = Network((n, n, n), scales, params=init_params(seed), train=False)
+    rng = np.random.default_rng(seed)
+    fields = []
+    for _ in range(3):
+        a = np.zeros((V, 4), dtype=np.float32)
+        a[:, :3] = rng.normal(0.0, 1e-3, size=(V, 3))
+        d = Dev(16 * V)
+        d.upload(a)
+        fields.append(d)
+    net.pack(fields[0].ptr, fields[1].ptr, fields[2].ptr)
+    for _ in range(int(reps)):
+        net.forward()
+    net.sync()
+
+
+if __name__ == "__main__":
+    import argparse
+    ap = argparse.ArgumentParser(description="inference passes of the operator on random fields (profiling)")
+    ap.add_argument("--n", type=int, default=64)
+    ap.add_argument("--reps", type=int, default=10)
+    a = ap.parse_args()
+    forward_passes(a.n, a.reps)
+    print(f"{a.reps} forward passes at {a.n}^3", flush=True)
