@@ -101,7 +101,8 @@ def pmc_summary(fetch_csv: str, write_csv: str, fetch_scale: float, write_scale:
            "FETCH_SIZE_kib_per_dispatch": fe, "WRITE_SIZE_kib_per_dispatch": wr,
            "fetch_scale": fetch_scale, "write_scale": write_scale,
            "note": "scales from a 1 GiB device copy (python bench.py --calibrate-copy under the same counters)",
-           "hbm_read_bytes_per_launch": rd, "hbm_write_bytes_per_launch": wb, "hbm_bytes_per_launch": rd + wb}
+           "hbm_read_bytes_per_launch": rd, "hbm_write_bytes_per_launch": wb, "hbm_bytes_per_launch": rd + wb,
+           "source_sha256_covers": "csrc/ipic_gpu.hip up to the field kernels banner (the push kernel device code)"}
     with open(out, "w") as f:
         json.dump(rec, f, indent=1)
     print(json.dumps(rec))
