@@ -123,7 +123,7 @@ struct PushArgs {
     const float4* Bn;
     float* jpart;
     long long* jover;     // overflow deposits, 64-bit fixed point like the tile accumulator
-    int* jflag;
+    unsigned char* jmark; // jmark[i] = 1: node i holds overflow deposits
     unsigned long long* removed;
     int* err;             // set to nonzero when a consistency guard fires
     const int* tile_order;  // block -> tile (a permutation), or null for the XCD-aware default
@@ -312,6 +312,7 @@ __device__ __forceinline__ void deposit_direct(const PushArgs& A, const TG& G, l
                     const int gid = nid_of(g, wrapc(x0 + elx + i, g.n[0]), wrapc(y0 + ely + j, g.n[1]),
                                            wrapc(z0 + elz + k, g.n[2]));
                     long long* o = A.jover + 3 * (long long)gid;
+                    A.jmark[gid] = 1;  // a plain byte store: every writer stores the same value
                     __hip_atomic_fetch_add(o + 0, fix_of(jx * ww, g.fix_scale), __ATOMIC_RELAXED,
                                            __HIP_MEMORY_SCOPE_AGENT);
                     __hip_atomic_fetch_add(o + 1, fix_of(jy * ww, g.fix_scale), __ATOMIC_RELAXED,
@@ -1184,7 +1185,7 @@ struct FieldArgs {
     const float4* SB;     // static B source (nullable)
     const float* jpart;
     long long* jover;
-    int* jflag;
+    unsigned char* jmark;
     const int* jpairs;  // [3][nmax][1+JPMAX]: count, then (tile << 8 | local)
     int nsl_read;       // slices of each tile's partials the sum reads: 1 once k_slice_sum ran
     float dt;           // field substep
@@ -1238,7 +1239,9 @@ __device__ __forceinline__ float4 sum_partials(const FieldArgs& F, int i, int x,
                     sz += (double)base[2 * g.NJ + l];
                 }
             }
-    if (*F.jflag) {
+    // the overflow accumulator, read and cleared only at the nodes that hold deposits
+    if (F.jmark[i]) {
+        F.jmark[i] = 0;
         long long* o = F.jover + 3 * (long long)i;
         sx += (double)o[0] * g.fix_inv;
         sy += (double)o[1] * g.fix_inv;
@@ -1274,14 +1277,15 @@ __global__ __launch_bounds__(256) void k_jsum(FieldArgs F) {
     F.J[i] = sum_partials(F, i, x, y, z);
 }
 
-__global__ void k_clear_flag(int* f) { *f = 0; }
-__global__ void k_set_flag(int* f) { *f = 1; }
-
 // Drops overflow deposits that were never summed (a deposit pass replaces the partials).
-__global__ void k_discard_overflow(const int* flag, long long* jover, long long n) {
-    if (!*flag) return;
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) jover[i] = 0;
+__global__ void k_discard_overflow(unsigned char* jmark, long long* jover, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !jmark[i]) return;
+    jmark[i] = 0;
+    long long* o = jover + 3 * (long long)i;
+    o[0] = 0;
+    o[1] = 0;
+    o[2] = 0;
 }
 
 // E pass of field.h:359-363: E += dt (c^2 curl- Bc - J).  With from_partials J is first
@@ -1321,11 +1325,10 @@ __global__ __launch_bounds__(256) void k_field_E(FieldArgs F, int from_partials)
     }
 }
 
-// B pass of field.h:366-371: Bc -= dt curl+ E.  Clears the overflow flag read by the E pass.
+// B pass of field.h:366-371: Bc -= dt curl+ E.
 __global__ __launch_bounds__(256) void k_field_B(FieldArgs F) {
     const Geo& g = F.g;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) *F.jflag = 0;
     if (i >= g.ncell) return;
     int x, y, z;
     xyz_of(g, i, x, y, z);
@@ -1800,7 +1803,7 @@ struct ipic_ctx {
     std::vector<double> bgE, bgBc;  // background (state at the last field upload), FP64
     float* jpart = nullptr;
     long long* jover = nullptr;
-    int* jflag = nullptr;
+    unsigned char* jmark = nullptr;  // nodes that hold overflow deposits (one byte each)
     int* jpairs = nullptr;
     unsigned long long* removed = nullptr;
     int* derr = nullptr;
@@ -1919,7 +1922,7 @@ PushArgs push_args(ipic_ctx* ctx) {
     A.Bn = ctx->Bn;
     A.jpart = ctx->jpart;
     A.jover = ctx->jover;
-    A.jflag = ctx->jflag;
+    A.jmark = ctx->jmark;
     A.removed = ctx->removed;
     A.err = ctx->derr;
     A.tile_order = ctx->order_valid ? ctx->tile_order : nullptr;
@@ -1941,7 +1944,7 @@ FieldArgs field_args(ipic_ctx* ctx) {
     F.SB = ctx->SB;
     F.jpart = ctx->jpart;
     F.jover = ctx->jover;
-    F.jflag = ctx->jflag;
+    F.jmark = ctx->jmark;
     F.jpairs = ctx->jpairs;
     F.nsl_read = ctx->g.nsl;
     F.dt = ctx->d.field_substeps > 0 ? (float)(ctx->d.dt / ctx->d.field_substeps) : 0.0f;
@@ -1991,10 +1994,8 @@ void launch_opt(int opt, dim3 grid, dim3 block, size_t lds, hipStream_t st, cons
 
 int launch_particles(ipic_ctx* ctx, int mode) {
     if (mode != MODE_REBIN && !ctx->j_summed) {
-        const long long n = 3LL * ctx->g.ncell;
-        hipLaunchKernelGGL(k_discard_overflow, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, ctx->jflag,
-                           ctx->jover, n);
-        hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(1), 0, ctx->stream, ctx->jflag);
+        hipLaunchKernelGGL(k_discard_overflow, dim3((unsigned)((ctx->g.ncell + 255) / 256)), dim3(256), 0,
+                           ctx->stream, ctx->jmark, ctx->jover, ctx->g.ncell);
     }
     PushArgs A = push_args(ctx);
     const Geo& g = ctx->g;
@@ -2014,10 +2015,6 @@ int launch_particles(ipic_ctx* ctx, int mode) {
         else if (mode == MODE_DEPOSIT) launch_opt<MODE_DEPOSIT, false>(opt, grid, block, lds, st, A);
         else launch_opt<MODE_REBIN, false>(opt, grid, block, lds, st, A);
     }
-    // A deposit pass may leave deposits in the global overflow accumulator (particles that cross
-    // a tile face): it is marked for the next J sum here, once, rather than by a store of every
-    // depositing lane to one shared word.
-    if (mode != MODE_REBIN) hipLaunchKernelGGL(k_set_flag, dim3(1), dim3(1), 0, st, ctx->jflag);
     HIPCK(ctx, hipGetLastError());
     if (mode != MODE_REBIN) ctx->j_summed = false;
     if (mode != MODE_DEPOSIT)
@@ -2087,7 +2084,6 @@ int launch_jsum(ipic_ctx* ctx) {
     if (int rc = slice_sum(ctx, F)) return rc;
     const int nb = (ctx->g.ncell + 255) / 256;
     hipLaunchKernelGGL(k_jsum, dim3(nb), dim3(256), 0, ctx->stream, F);
-    hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(1), 0, ctx->stream, ctx->jflag);
     HIPCK(ctx, hipGetLastError());
     ctx->j_summed = true;
     return 0;
@@ -2526,7 +2522,7 @@ int ipic_gpu_create(const ipic_desc* desc, ipic_ctx** out) {
         rc = rc ? rc : dalloc(ctx, &ctx->J, nc);
         rc = rc ? rc : dalloc(ctx, &ctx->jpart, (size_t)g.ntiles * g.nsl * 3 * g.NJ);
         rc = rc ? rc : dalloc(ctx, &ctx->jover, nc * 3);
-        rc = rc ? rc : dalloc(ctx, &ctx->jflag, 1);
+        rc = rc ? rc : dalloc(ctx, &ctx->jmark, nc);
         rc = rc ? rc : dalloc(ctx, &ctx->jpairs, jp.size());
         rc = rc ? rc : dalloc(ctx, &ctx->removed, 1);
         rc = rc ? rc : dalloc(ctx, &ctx->derr, 1);
