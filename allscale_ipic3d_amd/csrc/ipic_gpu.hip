@@ -1431,9 +1431,11 @@ __global__ __launch_bounds__(1024) void k_scan_top(ScanArgs A) {
     if (threadIdx.x == 1023) S.tile_base[g.ntiles] = s_part[1023];
 }
 
-// New starts = tile base + in-tile exclusive prefix of the counts; the previous starts
-// become the segments of the array just written; stayers move to `stay`; accumulators and
-// arrival counters are cleared for the next pass.
+// New starts = tile base + in-tile exclusive prefix of the counts, written over the old
+// segments; the host then swaps `seg` and `start` (the previous starts become the segments of
+// the array just written, without a copy) and `stay` and `stay_acc` (the stayer counts become
+// `stay`; the old `stay`, cleared here, the next accumulator).  The cell counters and arrival
+// counters are cleared for the next pass.
 __global__ __launch_bounds__(NT) void k_scan_cells(ScanArgs A) {
     const Geo& g = A.g;
     const ScanSp& S = A.sp[blockIdx.y];
@@ -1447,25 +1449,18 @@ __global__ __launch_bounds__(NT) void k_scan_cells(ScanArgs A) {
     int tot;
     const int ex = block_exscan(v0 + v1, s_w, tot) + S.tile_base[t];
     if (c0 < g.TC) {
-        S.seg[id0] = S.start[id0];
-        S.start[id0] = ex;
-        S.stay[id0] = S.stay_acc[id0];
-        S.stay_acc[id0] = 0;
+        S.seg[id0] = ex;
+        S.stay[id0] = 0;
         S.cnt_acc[id0] = 0;
         S.arr[id0] = 0;
     }
     if (c1 < g.TC) {
-        S.seg[id1] = S.start[id1];
-        S.start[id1] = ex + v0;
-        S.stay[id1] = S.stay_acc[id1];
-        S.stay_acc[id1] = 0;
+        S.seg[id1] = ex + v0;
+        S.stay[id1] = 0;
         S.cnt_acc[id1] = 0;
         S.arr[id1] = 0;
     }
-    if (t == g.ntiles - 1 && threadIdx.x == 0) {
-        S.seg[g.ncell] = S.start[g.ncell];
-        S.start[g.ncell] = S.tile_base[g.ntiles];
-    }
+    if (t == g.ntiles - 1 && threadIdx.x == 0) S.seg[g.ncell] = S.tile_base[g.ntiles];
 }
 
 // Launch order of the particle kernel's tiles for the layout the scans just produced.  A tile
@@ -2030,6 +2025,10 @@ int launch_scan(ipic_ctx* ctx) {
     hipLaunchKernelGGL(k_scan_cells, dim3(ctx->g.ntiles, ns), dim3(NT), 0, ctx->stream, A);
     hipLaunchKernelGGL(k_tile_order, dim3(1), dim3(1024), 0, ctx->stream, A, ns, ctx->tile_order);
     HIPCK(ctx, hipGetLastError());
+    for (int s = 0; s < ns; ++s) {  // the roles k_scan_cells gave the arrays
+        std::swap(ctx->sp[s].seg, ctx->sp[s].start);
+        std::swap(ctx->sp[s].stay, ctx->sp[s].stay_acc);
+    }
     ctx->order_valid = true;
     return 0;
 }
