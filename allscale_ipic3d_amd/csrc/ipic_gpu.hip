@@ -312,7 +312,8 @@ __device__ __forceinline__ void deposit_direct(const PushArgs& A, const TG& G, l
                     const int gid = nid_of(g, wrapc(x0 + elx + i, g.n[0]), wrapc(y0 + ely + j, g.n[1]),
                                            wrapc(z0 + elz + k, g.n[2]));
                     long long* o = A.jover + 3 * (long long)gid;
-                    A.jmark[gid] = 1;  // a plain byte store: every writer stores the same value
+                    // every writer stores the same value: a relaxed atomic byte store
+                    __hip_atomic_store(&A.jmark[gid], (unsigned char)1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     __hip_atomic_fetch_add(o + 0, fix_of(jx * ww, g.fix_scale), __ATOMIC_RELAXED,
                                            __HIP_MEMORY_SCOPE_AGENT);
                     __hip_atomic_fetch_add(o + 1, fix_of(jy * ww, g.fix_scale), __ATOMIC_RELAXED,
