@@ -1405,7 +1405,80 @@ __global__ __launch_bounds__(NT) void k_scan_tiles(ScanArgs A) {
     if (threadIdx.x == 0) S.tile_total[t] = tot;
 }
 
-__global__ __launch_bounds__(1024) void k_scan_top(ScanArgs A) {
+// Launch order of the particle kernel's tiles for the layout the scans just produced.  A
+// block's time grows with its tile's particles.  With an uneven load (the heaviest tile more
+// than 25 % above the mean) the heaviest tiles go first (longest-processing-time order over 256
+// load classes), so that the blocks that start last are light.  Otherwise the tiles go in a
+// scrambled order, tile (b * mul) mod ntiles for block b with mul coprime to ntiles: neighbouring
+// tiles then run at different times, which on the uniform 128^3 x 32 workload takes 2 % off the
+// push against launching neighbours together (the XCD-aware order).  Any permutation gives the
+// same results: the order only schedules.
+__device__ void tile_order_block(const ScanArgs& A, int nsp, int* order, int mul) {
+    const Geo& g = A.g;
+    __shared__ unsigned long long s_sum[16];
+    __shared__ int s_max[16], s_w4[4];
+    __shared__ int s_hist[256];
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    auto load = [&](int t) {
+        int v = 0;
+        for (int sp = 0; sp < nsp; ++sp) v += A.sp[sp].tile_total[t];
+        return v;
+    };
+    unsigned long long sum = 0;
+    int mx = 0;
+    for (int t = tid; t < g.ntiles; t += 1024) {
+        const int v = load(t);
+        sum += (unsigned long long)v;
+        mx = max(mx, v);
+    }
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) {
+        sum += __shfl_xor(sum, d, 64);
+        mx = max(mx, __shfl_xor(mx, d, 64));
+    }
+    if (lane == 0) {
+        s_sum[wid] = sum;
+        s_max[wid] = mx;
+    }
+    if (tid < 256) s_hist[tid] = 0;
+    __syncthreads();
+    sum = 0;
+    mx = 0;
+    for (int w = 0; w < 16; ++w) {
+        sum += s_sum[w];
+        mx = max(mx, s_max[w]);
+    }
+    if ((double)mx * g.ntiles <= 1.25 * (double)sum || mx == 0) {
+        for (int b = tid; b < g.ntiles; b += 1024) order[b] = (int)(((long long)b * mul) % g.ntiles);
+        return;
+    }
+    auto cls = [&](int v) { return 255 - (int)(((long long)v * 255) / mx); };  // heaviest: class 0
+    for (int t = tid; t < g.ntiles; t += 1024) atomicAdd(&s_hist[cls(load(t))], 1);
+    __syncthreads();
+    // exclusive prefix sum of the 256 class counts, by the first four waves
+    int c = 0, incl = 0;
+    if (tid < 256) {
+        c = s_hist[tid];
+        incl = wave_incl_scan(c, lane);
+        if (lane == 63) s_w4[wid] = incl;
+    }
+    __syncthreads();
+    if (tid < 256) {
+        int off = 0;
+        for (int w = 0; w < wid; ++w) off += s_w4[w];
+        s_hist[tid] = off + incl - c;
+    }
+    __syncthreads();
+    for (int t = tid; t < g.ntiles; t += 1024) order[atomicAdd(&s_hist[cls(load(t))], 1)] = t;
+}
+
+// Tile bases from the tile totals, one block per species; block nsp (the last) orders the
+// tiles for the next push meanwhile.
+__global__ __launch_bounds__(1024) void k_scan_top(ScanArgs A, int nsp, int* order, int mul) {
+    if ((int)blockIdx.x == nsp) {
+        tile_order_block(A, nsp, order, mul);
+        return;
+    }
     const Geo& g = A.g;
     const ScanSp& S = A.sp[blockIdx.x];
     __shared__ int s_part[1024];
@@ -1464,64 +1537,6 @@ __global__ __launch_bounds__(NT) void k_scan_cells(ScanArgs A) {
     if (t == g.ntiles - 1 && threadIdx.x == 0) S.seg[g.ncell] = S.tile_base[g.ntiles];
 }
 
-// Launch order of the particle kernel's tiles for the layout the scans just produced.  A tile
-// costs time in proportion to its particles, and one block works one tile, so with an uneven
-// load the heaviest tiles go first (longest-processing-time order, by 256 load classes); with
-// a load within 25 % of the mean the XCD-aware order stays.  Any permutation gives the same
-// results: the order only schedules.
-__global__ __launch_bounds__(1024) void k_tile_order(ScanArgs A, int nsp, int* order) {
-    const Geo& g = A.g;
-    __shared__ unsigned long long s_sum[16];
-    __shared__ int s_max[16];
-    __shared__ int s_hist[256];
-    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    auto load = [&](int t) {
-        int v = 0;
-        for (int sp = 0; sp < nsp; ++sp) v += A.sp[sp].tile_total[t];
-        return v;
-    };
-    unsigned long long sum = 0;
-    int mx = 0;
-    for (int t = tid; t < g.ntiles; t += 1024) {
-        const int v = load(t);
-        sum += (unsigned long long)v;
-        mx = max(mx, v);
-    }
-#pragma unroll
-    for (int d = 32; d > 0; d >>= 1) {
-        sum += __shfl_xor(sum, d, 64);
-        mx = max(mx, __shfl_xor(mx, d, 64));
-    }
-    if (lane == 0) {
-        s_sum[wid] = sum;
-        s_max[wid] = mx;
-    }
-    if (tid < 256) s_hist[tid] = 0;
-    __syncthreads();
-    sum = 0;
-    mx = 0;
-    for (int w = 0; w < 16; ++w) {
-        sum += s_sum[w];
-        mx = max(mx, s_max[w]);
-    }
-    if ((double)mx * g.ntiles <= 1.25 * (double)sum || mx == 0) {
-        for (int b = tid; b < g.ntiles; b += 1024) order[b] = xcd_tile(b, g.ntiles);
-        return;
-    }
-    auto cls = [&](int v) { return 255 - (int)(((long long)v * 255) / mx); };  // heaviest: class 0
-    for (int t = tid; t < g.ntiles; t += 1024) atomicAdd(&s_hist[cls(load(t))], 1);
-    __syncthreads();
-    if (tid == 0) {
-        int run = 0;
-        for (int k = 0; k < 256; ++k) {
-            const int c = s_hist[k];
-            s_hist[k] = run;
-            run += c;
-        }
-    }
-    __syncthreads();
-    for (int t = tid; t < g.ntiles; t += 1024) order[atomicAdd(&s_hist[cls(load(t))], 1)] = t;
-}
 
 // After a REBIN pass the array is sorted by current cell: segments = starts, all stay.
 __global__ __launch_bounds__(256) void k_canon_fix(ScanArgs A) {
@@ -1804,6 +1819,7 @@ struct ipic_ctx {
     unsigned long long* removed = nullptr;
     int* derr = nullptr;
     int* tile_order = nullptr;     // launch order of the particle kernel's tiles
+    int order_mul = 1;             // multiplier of the scrambled tile order, coprime to ntiles
     bool order_valid = false;      // tile_order describes the current layout's loads
     double* red = nullptr;
     int red_blocks = 1024;
@@ -2022,9 +2038,8 @@ int launch_scan(ipic_ctx* ctx) {
     ScanArgs A = scan_args(ctx);
     const int ns = ctx->d.nspecies;
     hipLaunchKernelGGL(k_scan_tiles, dim3(ctx->g.ntiles, ns), dim3(NT), 0, ctx->stream, A);
-    hipLaunchKernelGGL(k_scan_top, dim3(ns), dim3(1024), 0, ctx->stream, A);
+    hipLaunchKernelGGL(k_scan_top, dim3(ns + 1), dim3(1024), 0, ctx->stream, A, ns, ctx->tile_order, ctx->order_mul);
     hipLaunchKernelGGL(k_scan_cells, dim3(ctx->g.ntiles, ns), dim3(NT), 0, ctx->stream, A);
-    hipLaunchKernelGGL(k_tile_order, dim3(1), dim3(1024), 0, ctx->stream, A, ns, ctx->tile_order);
     HIPCK(ctx, hipGetLastError());
     for (int s = 0; s < ns; ++s) {  // the roles k_scan_cells gave the arrays
         std::swap(ctx->sp[s].seg, ctx->sp[s].start);
@@ -2508,6 +2523,22 @@ int ipic_gpu_create(const ipic_desc* desc, ipic_ctx** out) {
         if (build_jpairs(g, jp) != 0) {
             delete ctx;
             return fail(nullptr, IPIC_E_ARG, "tile/grid combination not supported");
+        }
+        {
+            // multiplier of the scrambled tile order: near 0.618 ntiles, so that consecutive
+            // blocks take tiles far apart, and coprime to ntiles, so that b -> b * mul mod
+            // ntiles is a permutation
+            auto gcd = [](int a, int b) {
+                while (b) {
+                    const int r = a % b;
+                    a = b;
+                    b = r;
+                }
+                return a;
+            };
+            int m = std::max(1, (int)(0.6180339887 * g.ntiles));
+            while (gcd(m, g.ntiles) != 1) ++m;
+            ctx->order_mul = m;
         }
         hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
         if (e != hipSuccess) {
