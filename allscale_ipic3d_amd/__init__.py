@@ -21,6 +21,7 @@ FLAG_REFLECT = 8
 FLAG_UNIFORM_BACKGROUND = 16
 FLAG_DIPOLE_MOVER = 32
 FLAG_FORWARD_SOLVER = 64
+FLAG_REPRODUCIBLE = 128
 
 
 class SpeciesDesc(C.Structure):

@@ -1551,6 +1551,102 @@ __global__ __launch_bounds__(256) void k_canon_fix(ScanArgs A) {
     }
 }
 
+// Reproducible order of each cell's arrivals (IPIC_FLAG_REPRODUCIBLE).  A pass that moves
+// particles writes the particles it takes into another cell (the arrivals of the lagged sort)
+// to that cell's slots [start + stay, start + stay + arr) in the order of atomic counters, which
+// follows timing.  These kernels reorder every such range by the particles' own contents (the
+// bit patterns of the three offsets and three velocities, then the id when ids are carried),
+// which do not depend on timing.  Every cell's order is then a function of the state, and so is
+// the next pass, whose runs (and through them J's FP32 moment sums) follow that order.
+struct CanonSp {
+    float* out;          // the array the pass wrote: six arrays of cap floats
+    float* spare;        // the array the pass read, dead now: holds a copy of each range
+    uint32_t* id_out;    // particle ids (null when not carried)
+    uint32_t* id_spare;
+    long long cap;
+    const int* start;
+    const int* stay;
+    const int* arr;
+    int* work;           // [0]: number of cells left to k_canon_large, then their ids
+};
+struct CanonArgs {
+    int ncell;
+    int* err;
+    CanonSp sp[IPIC_MAX_SPECIES];
+};
+
+constexpr int CANON_SMALL = 16;  // a cell with at most this many arrivals is ordered by one thread
+
+// Particle in slot a comes before the one in slot b (both read from the copy).  Slots decide
+// only between particles identical in every word, where either order gives the same array.
+__device__ __forceinline__ bool canon_before(const CanonSp& S, long long a, long long b) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        const uint32_t x = __float_as_uint(S.spare[k * S.cap + a]), y = __float_as_uint(S.spare[k * S.cap + b]);
+        if (x != y) return x < y;
+    }
+    if (S.id_spare) {
+        const uint32_t x = S.id_spare[a], y = S.id_spare[b];
+        if (x != y) return x < y;
+    }
+    return a < b;
+}
+
+__device__ __forceinline__ void canon_copy(const CanonSp& S, long long i) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) S.spare[k * S.cap + i] = S.out[k * S.cap + i];
+    if (S.id_out) S.id_spare[i] = S.id_out[i];
+}
+
+// The particle of copy slot i goes to its rank among the range [lo, lo + n).
+__device__ __forceinline__ void canon_place(const CanonSp& S, long long lo, int n, long long i) {
+    int r = 0;
+    for (int j = 0; j < n; ++j) r += canon_before(S, lo + j, i) ? 1 : 0;  // j == i counts 0
+#pragma unroll
+    for (int k = 0; k < 6; ++k) S.out[k * S.cap + lo + r] = S.spare[k * S.cap + i];
+    if (S.id_out) S.id_out[lo + r] = S.id_spare[i];
+}
+
+__global__ void k_canon_reset(CanonArgs A, int nsp) {
+    if ((int)threadIdx.x < nsp) A.sp[threadIdx.x].work[0] = 0;
+}
+
+// One thread per cell and species: cells with 2..CANON_SMALL arrivals are ordered here, larger
+// ones are listed for k_canon_large.
+__global__ __launch_bounds__(256) void k_canon_small(CanonArgs A) {
+    const CanonSp& S = A.sp[blockIdx.y];
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= A.ncell) return;
+    const int n = S.arr[c];
+    if (n < 2) return;
+    const long long lo = (long long)S.start[c] + S.stay[c];
+    if (lo < 0 || lo + n > S.start[c + 1] || lo + n > S.cap) {
+        atomicOr(A.err, ERR_SLOT);
+        return;
+    }
+    if (n > CANON_SMALL) {
+        S.work[1 + atomicAdd(S.work, 1)] = c;  // at most one entry per cell: ncell + 1 ints
+        return;
+    }
+    for (int i = 0; i < n; ++i) canon_copy(S, lo + i);
+    for (int i = 0; i < n; ++i) canon_place(S, lo, n, lo + i);
+}
+
+// One block per listed cell (grid-stride over the list): the range is copied, then each thread
+// places its share of the particles.  Ranges of different cells are disjoint.
+__global__ __launch_bounds__(256) void k_canon_large(CanonArgs A) {
+    const CanonSp& S = A.sp[blockIdx.y];
+    const int nwork = S.work[0];
+    for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
+        const int c = S.work[1 + w];
+        const int n = S.arr[c];
+        const long long lo = (long long)S.start[c] + S.stay[c];  // checked when listed
+        for (int i = threadIdx.x; i < n; i += blockDim.x) canon_copy(S, lo + i);
+        __syncthreads();
+        for (int i = threadIdx.x; i < n; i += blockDim.x) canon_place(S, lo, n, lo + i);
+    }
+}
+
 // ---------------------------------------------------------------------------------------------
 // synthetic initialisation and FP64 reductions
 // ---------------------------------------------------------------------------------------------
@@ -1793,6 +1889,7 @@ struct Species {
     int* arr = nullptr;
     int* tile_total = nullptr;
     int* tile_base = nullptr;
+    int* work = nullptr;  // cells whose arrivals k_canon_large orders (IPIC_FLAG_REPRODUCIBLE)
 };
 
 }  // namespace
@@ -2049,6 +2146,26 @@ int launch_scan(ipic_ctx* ctx) {
     return 0;
 }
 
+// After a pass that moved particles (the buffers already swapped): each cell's arrivals in the
+// order of their contents, when the context asks for reproducible steps.
+int launch_canon(ipic_ctx* ctx) {
+    if (!(ctx->d.flags & IPIC_FLAG_REPRODUCIBLE)) return 0;
+    CanonArgs A{};
+    const int ns = ctx->d.nspecies;
+    A.ncell = ctx->g.ncell;
+    A.err = ctx->derr;
+    for (int s = 0; s < ns; ++s) {
+        Species& S = ctx->sp[s];
+        A.sp[s] = CanonSp{S.X[S.cur], S.X[1 - S.cur], S.ids[S.cur], S.ids[1 - S.cur], S.cap,
+                          S.start, S.stay, S.arr, S.work};
+    }
+    hipLaunchKernelGGL(k_canon_reset, dim3(1), dim3(64), 0, ctx->stream, A, ns);
+    hipLaunchKernelGGL(k_canon_small, dim3((unsigned)((ctx->g.ncell + 255) / 256), ns), dim3(256), 0, ctx->stream, A);
+    hipLaunchKernelGGL(k_canon_large, dim3(256, ns), dim3(256), 0, ctx->stream, A);
+    HIPCK(ctx, hipGetLastError());
+    return 0;
+}
+
 int launch_jsum(ipic_ctx* ctx);
 
 // Reduces the block slices of the partials (a no-op with one slice per tile); F reads one slice
@@ -2241,6 +2358,7 @@ int particle_phase(ipic_ctx* ctx, ipic_step_stats* st) {
     if (timing) HIPCK(ctx, hipEventRecord(ctx->ev[4], ctx->stream));
     if (int rc = launch_particles(ctx, MODE_PUSH)) return rc;
     if (timing) HIPCK(ctx, hipEventRecord(ctx->ev[5], ctx->stream));
+    if (int rc = launch_canon(ctx)) return rc;
     if (int rc = launch_scan(ctx)) return rc;
     if (timing) {
         HIPCK(ctx, hipEventRecord(ctx->ev[6], ctx->stream));
@@ -2259,6 +2377,7 @@ int check_device_flag(ipic_ctx* ctx);
 int canonicalize(ipic_ctx* ctx) {
     if (ctx->canonical) return 0;
     int rc = launch_particles(ctx, MODE_REBIN);
+    if (!rc) rc = launch_canon(ctx);
     if (rc) return rc;
     ScanArgs A = scan_args(ctx);
     hipLaunchKernelGGL(k_canon_fix, dim3((ctx->g.ncell + 1 + 255) / 256, ctx->d.nspecies), dim3(256), 0,
@@ -2571,6 +2690,7 @@ int ipic_gpu_create(const ipic_desc* desc, ipic_ctx** out) {
             rc = rc ? rc : dalloc(ctx, &S.arr, nc);
             rc = rc ? rc : dalloc(ctx, &S.tile_total, (size_t)g.ntiles);
             rc = rc ? rc : dalloc(ctx, &S.tile_base, (size_t)g.ntiles + 1);
+            if (d.flags & IPIC_FLAG_REPRODUCIBLE) rc = rc ? rc : dalloc(ctx, &S.work, nc + 1);
         }
         if (!rc) {
             e = hipMemcpyAsync(ctx->jpairs, jp.data(), jp.size() * sizeof(int), hipMemcpyHostToDevice, ctx->stream);

@@ -46,6 +46,11 @@ extern "C" {
                                      ipic_desc.dipole_moment; the field grids are not gathered */
 #define IPIC_FLAG_FORWARD_SOLVER 64u /* field policy: the explicit forward solver (solveFieldForward,
                                      field.h:293-337) instead of leapfrog */
+#define IPIC_FLAG_REPRODUCIBLE 128u /* after every pass that moves particles, each cell's arrivals
+                                     (particles that changed cell) are put in the order of their
+                                     contents rather than the order of atomic slot counters, so
+                                     steps are bitwise reproducible from run to run; the reference's
+                                     import (cell.h:1272-1306) is sequential, hence fixed, too */
 #define IPIC_FLAG_UNIFORM_BACKGROUND 16u /* upload_fields keeps only the uniform part of the
                                      fields as static background; the rest becomes the evolving
                                      FP32 state (needed by the field operator) */
