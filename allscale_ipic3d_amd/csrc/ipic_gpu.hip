@@ -1628,22 +1628,118 @@ __global__ __launch_bounds__(256) void k_canon_small(CanonArgs A) {
         S.work[1 + atomicAdd(S.work, 1)] = c;  // at most one entry per cell: ncell + 1 ints
         return;
     }
-    for (int i = 0; i < n; ++i) canon_copy(S, lo + i);
-    for (int i = 0; i < n; ++i) canon_place(S, lo, n, lo + i);
+    // the range in registers (every index below is a constant after unrolling), ranked there and
+    // written back in rank order: all loads precede the first store
+    uint32_t v[CANON_SMALL][7];
+#pragma unroll
+    for (int j = 0; j < CANON_SMALL; ++j)
+        if (j < n) {
+#pragma unroll
+            for (int k = 0; k < 6; ++k) v[j][k] = __float_as_uint(S.out[k * S.cap + lo + j]);
+            v[j][6] = S.id_out ? S.id_out[lo + j] : 0u;
+        }
+#pragma unroll
+    for (int i = 0; i < CANON_SMALL; ++i)
+        if (i < n) {
+            int r = 0;
+#pragma unroll
+            for (int j = 0; j < CANON_SMALL; ++j)
+                if (j < n && j != i) {
+                    bool before = j < i;  // identical in every word: the slots decide
+#pragma unroll
+                    for (int k = 6; k >= 0; --k)
+                        if (v[j][k] != v[i][k]) before = v[j][k] < v[i][k];
+                    r += before ? 1 : 0;
+                }
+#pragma unroll
+            for (int k = 0; k < 6; ++k) S.out[k * S.cap + lo + r] = __uint_as_float(v[i][k]);
+            if (S.id_out) S.id_out[lo + r] = v[i][6];
+        }
 }
 
-// One block per listed cell (grid-stride over the list): the range is copied, then each thread
-// places its share of the particles.  Ranges of different cells are disjoint.
+constexpr int CANON_LDS = 2048;  // arrivals whose leading key words k_canon_large keeps in LDS
+
+// One block per listed cell (grid-stride over the list); ranges of different cells are disjoint.
+// Up to CANON_LDS arrivals: the range is copied, the first two key words (x and y offset bits,
+// one 64-bit word whose unsigned order is theirs) are sorted with their slots in LDS by a
+// bitonic network, every group of equal words is then put in full-key order by one thread
+// (insertion sort; such groups are rare and the network leaves them in an order that follows the
+// input), and the particles are written back in sorted order.  Larger ranges rank each particle
+// against all others in global memory.
 __global__ __launch_bounds__(256) void k_canon_large(CanonArgs A) {
+    __shared__ unsigned long long s_key[CANON_LDS];
+    __shared__ int s_idx[CANON_LDS];
     const CanonSp& S = A.sp[blockIdx.y];
     const int nwork = S.work[0];
+    const int tid = threadIdx.x;
     for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
         const int c = S.work[1 + w];
         const int n = S.arr[c];
         const long long lo = (long long)S.start[c] + S.stay[c];  // checked when listed
-        for (int i = threadIdx.x; i < n; i += blockDim.x) canon_copy(S, lo + i);
+        if (n > CANON_LDS) {
+            for (int i = tid; i < n; i += blockDim.x) canon_copy(S, lo + i);
+            __syncthreads();
+            for (int i = tid; i < n; i += blockDim.x) canon_place(S, lo, n, lo + i);
+            __syncthreads();
+            continue;
+        }
+        int np = 1;
+        while (np < n) np <<= 1;
+        for (int i = tid; i < np; i += blockDim.x) {
+            if (i < n) {
+                canon_copy(S, lo + i);
+                s_key[i] = ((unsigned long long)__float_as_uint(S.out[lo + i]) << 32) |
+                           __float_as_uint(S.out[S.cap + lo + i]);
+            } else {
+                s_key[i] = ~0ull;  // padding: after every real entry (the slot breaks a tie)
+            }
+            s_idx[i] = i;
+        }
         __syncthreads();
-        for (int i = threadIdx.x; i < n; i += blockDim.x) canon_place(S, lo, n, lo + i);
+        // ascending in (leading words, slot), all pairs distinct
+        for (int k = 2; k <= np; k <<= 1)
+            for (int j = k >> 1; j > 0; j >>= 1) {
+                for (int i = tid; i < np; i += blockDim.x) {
+                    const int p = i ^ j;
+                    if (p > i) {
+                        const unsigned long long a = s_key[i], b = s_key[p];
+                        const bool asc = (i & k) == 0;
+                        const bool gt = a > b || (a == b && s_idx[i] > s_idx[p]);
+                        if (gt == asc) {
+                            s_key[i] = b;
+                            s_key[p] = a;
+                            const int t = s_idx[i];
+                            s_idx[i] = s_idx[p];
+                            s_idx[p] = t;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        // groups of equal leading words: full-key insertion sort by the group's first thread
+        for (int i = tid; i < n; i += blockDim.x) {
+            if ((i == 0 || s_key[i - 1] != s_key[i]) && i + 1 < n && s_key[i + 1] == s_key[i]) {
+                int e = i + 1;
+                while (e < n && s_key[e] == s_key[i]) ++e;
+                for (int a = i + 1; a < e; ++a) {
+                    const int v = s_idx[a];
+                    int b = a - 1;
+                    while (b >= i && canon_before(S, lo + v, lo + s_idx[b])) {
+                        s_idx[b + 1] = s_idx[b];
+                        --b;
+                    }
+                    s_idx[b + 1] = v;
+                }
+            }
+        }
+        __syncthreads();
+        for (int i = tid; i < n; i += blockDim.x) {
+            const long long src = lo + s_idx[i];
+#pragma unroll
+            for (int k = 0; k < 6; ++k) S.out[k * S.cap + lo + i] = S.spare[k * S.cap + src];
+            if (S.id_out) S.id_out[lo + i] = S.id_spare[src];
+        }
+        __syncthreads();  // s_key and s_idx are rewritten for the next cell
     }
 }
 
@@ -2161,7 +2257,7 @@ int launch_canon(ipic_ctx* ctx) {
     }
     hipLaunchKernelGGL(k_canon_reset, dim3(1), dim3(64), 0, ctx->stream, A, ns);
     hipLaunchKernelGGL(k_canon_small, dim3((unsigned)((ctx->g.ncell + 255) / 256), ns), dim3(256), 0, ctx->stream, A);
-    hipLaunchKernelGGL(k_canon_large, dim3(256, ns), dim3(256), 0, ctx->stream, A);
+    hipLaunchKernelGGL(k_canon_large, dim3(2048, ns), dim3(256), 0, ctx->stream, A);
     HIPCK(ctx, hipGetLastError());
     return 0;
 }

@@ -97,3 +97,72 @@ def test_crowded_arrivals_bitwise_equal_and_match_oracle():
     _compare_particles(u, a_sim, [-1.0], n, (1, 1, 1), 0.6)
     a_sim.close()
     u.close()
+
+
+def test_shipped_policies_bitwise_equal_and_match_oracle():
+    """The reference's default composition (dipole mover, reflection at the global faces, planet
+    sink, static field; simulator.h:244-276) with reproducible order: removed and reflected
+    arrivals go through the ordering too."""
+    n, origin = (8, 8, 8), (-4.0, -4.0, -4.0)
+    R, B1z, dt, q, qom = 1.0, 0.5, 0.15, -0.01, -25.0
+    rng = np.random.default_rng(31)
+    pos = rng.uniform(-4.0, 4.0, size=(40000, 3))
+    pos = pos[np.sum(pos * pos, axis=1) > (1.2 * R) ** 2]
+    vel = rng.normal(0.0, 0.2, size=(len(pos), 3)) + np.array([0.05, 0.0, 0.0])
+
+    def gpu():
+        sim = G.Simulation(n, origin=origin, dt=dt, species=((q, qom),), field_substeps=0,
+                           sink_center=(0.0, 0.0, 0.0), sink_radius=R, track_ids=True,
+                           flags=G.FLAG_REFLECT | G.FLAG_REPRODUCIBLE, dipole_moment=-B1z * R ** 3)
+        sim.upload_particles(0, pos, vel)
+        st = sim.step(5)
+        return sim, st
+
+    a_sim, st = gpu()
+    b_sim, _ = gpu()
+    assert st["removed"] > 0
+    _assert_bitwise_equal(_snapshot(a_sim, 1), _snapshot(b_sim, 1))
+    b_sim.close()
+    u = O.Universe(n, origin=origin, dt=dt, field_substeps=0, sink_center=(0.0, 0.0, 0.0), sink_radius=R,
+                   bc="reflect", mover="dipole", dipole_moment=-B1z * R ** 3)
+    u.add_particles(O.make_particles(pos, vel, q, qom))
+    u.step(5)
+    _compare_particles(u, a_sim, [qom], n, (1, 1, 1), 0.2)
+    np.testing.assert_array_equal(a_sim.cell_counts(), u.cell_counts())
+    a_sim.close()
+    u.close()
+
+
+def test_reference_driver_case_bitwise_equal_and_near_oracle():
+    """The reference's main driver case (inputs/test.inp: 262,144 particles, 1,000+ per populated
+    cell, about a hundred arrivals per cell and step) for its 30 steps, twice with reproducible
+    order: the runs agree bit for bit, and the counts stay within the default path's bound of the
+    FP64 oracle (a particle that FP32 and FP64 place on either side of a face)."""
+    import os
+    from allscale_ipic3d_amd import driver as D
+    gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    case = D.driver_case(D.Parameters(os.path.join(gold, "test_case.inp")), "test")
+
+    def gpu():
+        sim = D.make_simulation(case, flags=G.FLAG_REPRODUCIBLE)
+        sim.init_reference(0, D.case_distribution(case))
+        sim.step(case.ncycles)
+        return sim
+
+    a_sim, b_sim = gpu(), gpu()
+    _assert_bitwise_equal(_snapshot(a_sim, 1), _snapshot(b_sim, 1))
+    b_sim.close()
+    u = O.Universe(case.n, w=case.cell_width, origin=case.origin, dt=case.dt, c=case.c,
+                   sink_center=case.object_center, sink_radius=case.planet_radius, bc="reflect", mover="dipole",
+                   dipole_moment=case.dipole_moment, field_substeps=0)
+    _, parts = O.ref_init(u, case.pos_min, case.pos_max, (0, 0, 0), (case.v_mod,) * 3, case.num_particles,
+                          *case.species)
+    u.add_particles(parts)
+    for _ in range(case.ncycles):
+        u.move(8)
+        u.rebin(8)
+    gc, oc = a_sim.cell_counts(), u.cell_counts()
+    print("L1 count difference against the oracle after", case.ncycles, "steps:", int(np.abs(gc - oc).sum()))
+    assert np.abs(gc - oc).sum() / 2 <= 1e-4 * case.num_particles
+    a_sim.close()
+    u.close()
