@@ -212,6 +212,40 @@ __global__ __launch_bounds__(512) void k_conv(ConvArgs a, int nbricks) {
             }
 
         // D[row = z (4 kq + j)][col = output channel (t 16 + m)]
+        if constexpr (NT == 2) {
+            // lanes m and m ^ 1 swap one value (DPP quad permute 1,0,3,2): the even lane then
+            // holds channels m, m + 1 and the odd lane 15 + m, 16 + m of the same voxel, and
+            // each stores its pair at once, so a store instruction covers each voxel's whole
+            // 32-channel row (128 B in FP32, 64 B in BF16) instead of two halves
+            const bool odd = (m & 1) != 0;
+            const long long vb0 = (((long long)(x0 + w) * n1 + y0) * n2 + z0 + kq * 4) * COUT;  // channel 0
+            const float b0 = a.bias ? a.bias[m] : 0.0f, b1 = a.bias ? a.bias[16 + m] : 0.0f;
+#pragma unroll
+            for (int y = 0; y < BY; ++y)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const long long vb = vb0 + ((long long)y * n2 + j) * COUT;
+                    float v0 = acc[y][0][j] + b0, v1 = acc[y][1][j] + b1;
+                    if (a.zs) {
+                        v0 *= dsilu(a.zs[vb + m]);
+                        v1 *= dsilu(a.zs[vb + 16 + m]);
+                    }
+                    if (a.res) {
+                        v0 += a.res[vb + m];
+                        v1 += a.res[vb + 16 + m];
+                    }
+                    const float give = odd ? v0 : v1;
+                    const float got =
+                        __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(give), 0xB1, 0xF, 0xF, false));
+                    const float plo = odd ? got : v0, phi = odd ? v1 : got;
+                    const long long o = vb + (odd ? 15 + m : m);  // even: 8-byte aligned
+                    if (a.outf) *(float2*)(a.outf + o) = make_float2(plo, phi);
+                    if (a.outb)
+                        *(uint32_t*)(a.outb + o) = (uint32_t)f2bf(a.act ? silu(plo) : plo) |
+                                                   ((uint32_t)f2bf(a.act ? silu(phi) : phi) << 16);
+                }
+            continue;
+        }
         const long long ob = (((long long)(x0 + w) * n1 + y0) * n2 + z0 + kq * 4) * COUT + m;  // (t, y, j) = 0
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
