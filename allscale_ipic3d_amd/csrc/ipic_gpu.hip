@@ -1278,6 +1278,45 @@ __global__ __launch_bounds__(256) void k_jsum(FieldArgs F) {
     F.J[i] = sum_partials(F, i, x, y, z);
 }
 
+// BF16 of a finite float, rounded to nearest even (the network input's rounding).
+__device__ __forceinline__ uint32_t bf16_rne(float f) {
+    const uint32_t b = __float_as_uint(f);
+    return (b + 0x7FFFu + ((b >> 16) & 1u)) >> 16;
+}
+
+// The field operator's inputs in one pass: J from the tile partials when they are newer
+// (stored, as k_jsum), the state before the phase (Eb, Bb: the fallback and the checks start
+// from it), and the network input u, 16 BF16 channels per node (E iE, Bc iB, J iJ, zeros), in
+// the layout of ipic_nop_pack.
+__global__ __launch_bounds__(256) void k_op_prep(FieldArgs F, int from_partials, float4* Eb, float4* Bb, float iE,
+                                                 float iB, float iJ, uint16_t* u) {
+    const Geo& g = F.g;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= g.ncell) return;
+    float4 j;
+    if (from_partials) {
+        int x, y, z;
+        xyz_of(g, i, x, y, z);
+        j = sum_partials(F, i, x, y, z);
+        F.J[i] = j;
+    } else {
+        j = F.J[i];
+    }
+    const float4 e = F.E[i], b = F.Bc[i];
+    Eb[i] = e;
+    Bb[i] = b;
+    uint4 lo, hi;
+    lo.x = bf16_rne(e.x * iE) | (bf16_rne(e.y * iE) << 16);
+    lo.y = bf16_rne(e.z * iE) | (bf16_rne(b.x * iB) << 16);
+    lo.z = bf16_rne(b.y * iB) | (bf16_rne(b.z * iB) << 16);
+    lo.w = bf16_rne(j.x * iJ) | (bf16_rne(j.y * iJ) << 16);
+    hi.x = bf16_rne(j.z * iJ);
+    hi.y = hi.z = hi.w = 0u;
+    uint4* d = (uint4*)(u + (long long)i * 16);
+    d[0] = lo;
+    d[1] = hi;
+}
+
 // Drops overflow deposits that were never summed (a deposit pass replaces the partials).
 __global__ void k_discard_overflow(unsigned char* jmark, long long* jover, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -2024,6 +2063,8 @@ struct ipic_ctx {
     std::vector<void*> allocs;
     std::string err;
     hipEvent_t ev[8] = {};
+    hipEvent_t ev_wait = nullptr;  // polled by wait_stream
+    double* hred = nullptr;        // pinned host copy of the operator checks' block sums
     // field operator
     bool op_on = false;
     ipic_field_operator op{};      // settings (params pointer not kept)
@@ -2322,6 +2363,18 @@ int copy_f4(ipic_ctx* ctx, float4* dst, const float4* src) {
     return 0;
 }
 
+// Waits for the work queued on the context's stream by polling an event.  The operator phase
+// waits in the middle of a field phase for its check sums; a blocking wait can leave the host
+// thread asleep well past the end of the work, which then delays everything queued after it.
+int wait_stream(ipic_ctx* ctx) {
+    HIPCK(ctx, hipEventRecord(ctx->ev_wait, ctx->stream));
+    hipError_t e;
+    while ((e = hipEventQuery(ctx->ev_wait)) == hipErrorNotReady) {
+    }
+    HIPCK(ctx, e);
+    return 0;
+}
+
 int nop_fail(ipic_ctx* ctx, int rc) { return fail(ctx, rc, std::string("field operator: ") + ipic_nop_last_error()); }
 
 // One operator application with its checks, fallback and audit (the field phase without C2N).
@@ -2329,14 +2382,19 @@ int operator_phase(ipic_ctx* ctx, ipic_step_stats* st) {
     const Geo& g = ctx->g;
     if (ctx->SE || ctx->SB || ctx->Ebg || ctx->Bcbg)
         return fail(ctx, IPIC_E_STATE, "the field operator needs a uniform background field");
-    if (int rc = launch_jsum(ctx)) return rc;
-    if (int rc = copy_f4(ctx, ctx->Eb, ctx->E)) return rc;
-    if (int rc = copy_f4(ctx, ctx->Bb, ctx->Bc)) return rc;
+    {  // J (when the partials are newer), the state before the phase and the network input
+        FieldArgs F0 = field_args(ctx);
+        const int from = ctx->j_summed ? 0 : 1;
+        if (from)
+            if (int rc = slice_sum(ctx, F0)) return rc;
+        const ipic_nop_scales& sc = ctx->op_scales;
+        hipLaunchKernelGGL(k_op_prep, dim3((unsigned)((g.ncell + 255) / 256)), dim3(256), 0, ctx->stream, F0, from,
+                           ctx->Eb, ctx->Bb, 1.0f / sc.in_E, 1.0f / sc.in_B, 1.0f / sc.in_J, ctx->op_u);
+        HIPCK(ctx, hipGetLastError());
+        ctx->j_summed = true;
+    }
     const int32_t n3[3] = {g.n[0], g.n[1], g.n[2]};
     const float iw[3] = {g.inv_w[0], g.inv_w[1], g.inv_w[2]};
-    if (int rc = ipic_nop_pack(n3, (const float*)ctx->E, (const float*)ctx->Bc, (const float*)ctx->J, &ctx->op_scales,
-                               ctx->op_u, ctx->stream))
-        return nop_fail(ctx, rc);
     if (int rc = ipic_nop_forward(n3, ctx->op_u, ctx->op_wf, ctx->op_params, ctx->op_ws, 0, ctx->op_out, ctx->stream))
         return nop_fail(ctx, rc);
     if (int rc = ipic_nop_apply(n3, ctx->op_out, &ctx->op_scales, iw, (float*)ctx->E, (float*)ctx->Bc,
@@ -2346,9 +2404,10 @@ int operator_phase(ipic_ctx* ctx, ipic_step_stats* st) {
     FieldArgs F = field_args(ctx);
     hipLaunchKernelGGL(k_op_check, dim3(ctx->red_blocks), dim3(256), 0, ctx->stream, F, ctx->Eb, ctx->Bb, ctx->red);
     HIPCK(ctx, hipGetLastError());
-    std::vector<double> h((size_t)ctx->red_blocks * 5);
-    HIPCK(ctx, hipMemcpyAsync(h.data(), ctx->red, h.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-    HIPCK(ctx, hipStreamSynchronize(ctx->stream));
+    const double* h = ctx->hred;
+    HIPCK(ctx, hipMemcpyAsync(ctx->hred, ctx->red, (size_t)ctx->red_blocks * 5 * sizeof(double), hipMemcpyDeviceToHost,
+                              ctx->stream));
+    if (int rc = wait_stream(ctx)) return rc;
     double wo = 0, wn = 0, ejo = 0, ejn = 0, md = 0;
     for (int b = 0; b < ctx->red_blocks; ++b) {
         wo += h[5 * b];
@@ -2382,9 +2441,9 @@ int operator_phase(ipic_ctx* ctx, ipic_step_stats* st) {
         hipLaunchKernelGGL(k_audit, dim3(ctx->red_blocks), dim3(256), 0, ctx->stream, g, F.c2, ctx->Eo, ctx->Bo,
                            ctx->E, ctx->Bc, ctx->Eb, ctx->Bb, ctx->red);
         HIPCK(ctx, hipGetLastError());
-        HIPCK(ctx, hipMemcpyAsync(h.data(), ctx->red, (size_t)ctx->red_blocks * 4 * sizeof(double),
+        HIPCK(ctx, hipMemcpyAsync(ctx->hred, ctx->red, (size_t)ctx->red_blocks * 4 * sizeof(double),
                                   hipMemcpyDeviceToHost, ctx->stream));
-        HIPCK(ctx, hipStreamSynchronize(ctx->stream));
+        if (int rc = wait_stream(ctx)) return rc;
         double num = 0, den = 0, tot = 0, evo = 0;
         for (int b = 0; b < ctx->red_blocks; ++b) {
             num += h[4 * b];
@@ -2796,6 +2855,10 @@ int ipic_gpu_create(const ipic_desc* desc, ipic_ctx** out) {
         if (!rc)
             for (int k = 0; k < 8; ++k)
                 if (hipEventCreate(&ctx->ev[k]) != hipSuccess) rc = fail(ctx, IPIC_E_HIP, "hipEventCreate");
+        if (!rc && hipEventCreateWithFlags(&ctx->ev_wait, hipEventDisableTiming) != hipSuccess)
+            rc = fail(ctx, IPIC_E_HIP, "hipEventCreateWithFlags");
+        if (!rc && hipHostMalloc((void**)&ctx->hred, (size_t)ctx->red_blocks * 5 * sizeof(double)) != hipSuccess)
+            rc = fail(ctx, IPIC_E_HIP, "hipHostMalloc");
         if (rc) {
             std::string msg = ctx->err;
             ipic_gpu_destroy(ctx);
@@ -2814,6 +2877,8 @@ int ipic_gpu_destroy(ipic_ctx* ctx) {
         for (void* p : ctx->allocs) (void)hipFree(p);
         for (int k = 0; k < 8; ++k)
             if (ctx->ev[k]) (void)hipEventDestroy(ctx->ev[k]);
+        if (ctx->ev_wait) (void)hipEventDestroy(ctx->ev_wait);
+        if (ctx->hred) (void)hipHostFree(ctx->hred);
         if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
         delete ctx;
         return 0;
