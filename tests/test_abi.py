@@ -40,6 +40,18 @@ def test_library_exports_every_operator_symbol():
         assert hasattr(lib, name), name
 
 
+def test_flag_constants_match_header():
+    # every IPIC_FLAG_* of the header has the same value in the Python binding, and the bits
+    # are distinct
+    txt = open(os.path.join(ROOT, "include", "ipic_gpu.h")).read()
+    flags = {m.group(1): int(m.group(2)) for m in re.finditer(r"#define IPIC_(FLAG_\w+)\s+(\d+)u", txt)}
+    assert "FLAG_REPRODUCIBLE" in flags and len(flags) >= 7
+    for name, value in flags.items():
+        assert getattr(G, name) == value, name
+    values = list(flags.values())
+    assert all(v & (v - 1) == 0 for v in values) and len(set(values)) == len(values)
+
+
 def test_missing_library_raises(tmp_path):
     saved = G._lib
     G._lib = None
