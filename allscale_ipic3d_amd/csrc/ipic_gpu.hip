@@ -1614,7 +1614,7 @@ struct CanonArgs {
     CanonSp sp[IPIC_MAX_SPECIES];
 };
 
-constexpr int CANON_SMALL = 16;  // a cell with at most this many arrivals is ordered by one thread
+constexpr int CANON_SMALL = 4;  // a cell with at most this many arrivals is ordered by one thread
 
 // Particle in slot a comes before the one in slot b (both read from the copy).  Slots decide
 // only between particles identical in every word, where either order gives the same array.

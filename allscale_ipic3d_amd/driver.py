@@ -28,7 +28,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from . import FLAG_REFLECT, Simulation, desc_for, field_init, field_init_host, ref_distribution
+from . import FLAG_REFLECT, FLAG_REPRODUCIBLE, Simulation, desc_for, field_init, field_init_host, ref_distribution
 from . import io as ipic_io
 
 E_CHARGE = 1.602176565e-19   # elementary charge, C (ipic3d.cxx:70)
@@ -161,13 +161,15 @@ def case_distribution(case: DriverCase, nthreads=0):
                             seed=0, nthreads=nthreads)
 
 
-def run(config, out_dir=".", ncycles=None, write=True, tile=(0, 0, 0), verbose=False):
+def run(config, out_dir=".", ncycles=None, write=True, tile=(0, 0, 0), verbose=False, reproducible=False):
     """Runs a configuration as the reference's driver does; returns the final per-cell counts
-    (row-major), the conserved-quantity rows and the timings."""
+    (row-major), the conserved-quantity rows and the timings.  reproducible: each cell's
+    arrivals in the order of their contents (IPIC_FLAG_REPRODUCIBLE), so that two runs write
+    identical files."""
     base = os.path.splitext(os.path.basename(config))[0]
     case = driver_case(Parameters(config), base)
     steps = case.ncycles if ncycles is None else int(ncycles)
-    sim = make_simulation(case, tile=tile)
+    sim = make_simulation(case, tile=tile, flags=FLAG_REPRODUCIBLE if reproducible else 0)
     t0 = time.perf_counter()
     sim.init_reference(0, case_distribution(case))
     t_init = time.perf_counter() - t0
@@ -207,8 +209,10 @@ def main(argv=None):
     ap.add_argument("config")
     ap.add_argument("--out-dir", default=".")
     ap.add_argument("--cycles", type=int, default=None)
+    ap.add_argument("--reproducible", action="store_true",
+                    help="order each cell's arrivals by their contents: runs are bitwise reproducible")
     a = ap.parse_args(argv)
-    run(a.config, out_dir=a.out_dir, ncycles=a.cycles, verbose=True)
+    run(a.config, out_dir=a.out_dir, ncycles=a.cycles, verbose=True, reproducible=a.reproducible)
     return 0
 
 
