@@ -166,3 +166,16 @@ def test_reference_driver_case_bitwise_equal_and_near_oracle():
     assert np.abs(gc - oc).sum() / 2 <= 1e-4 * case.num_particles
     a_sim.close()
     u.close()
+
+
+def test_driver_runs_write_identical_files(tmp_path):
+    """The driver's --reproducible runs of the reference's test case (30 cycles) write
+    byte-identical count and conserved-quantity files."""
+    import os
+    from allscale_ipic3d_amd import driver as D
+    cfg = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "test_case.inp")
+    for d in ("a", "b"):
+        D.run(cfg, out_dir=str(tmp_path / d), reproducible=True)
+    for name in ("test_case.out", "test_caseConservedQuantities.out"):
+        a = (tmp_path / "a" / name).read_bytes()
+        assert len(a) > 0 and a == (tmp_path / "b" / name).read_bytes(), name
